@@ -1,0 +1,174 @@
+/*
+ * rgsim.h -- C ABI of the MI355X gravity solver (librgsim.so).
+ *
+ * Conventions
+ *  - Every entry point returns an int status (RG_OK = 0).  On failure,
+ *    rg_last_error() returns a thread-local message.  Nothing throws across
+ *    the ABI and nothing truncates silently.
+ *  - Array arguments are DEVICE pointers (e.g. torch tensor data_ptr()) in
+ *    the reference's layout: positions/velocities/accelerations are
+ *    (n, 3) row-major FP64, masses (n,) FP64.  The caller owns them.
+ *  - `stream` is a hipStream_t (NULL = the default stream).  Calls are
+ *    stream-ordered and asynchronous unless stated otherwise.
+ *  - Scratch memory: a BH handle owns its scratch.  rg_energy and
+ *    rg_morton_codes use a per-device library buffer, so two of those calls
+ *    that overlap on one device (different streams, host threads) race.
+ *  - `precision`: RG_FP64 reproduces the reference arithmetic; RG_FP32 is
+ *    plain FP32 pair arithmetic; RG_FP32C is FP32 pair arithmetic on
+ *    double-float (hi+lo) positions with compensated accumulation.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *  core/solvers/gravity.py, core/integrators/leapfrog.py,
+ *  core/solvers/barnes_hut.py, core/engine.py.
+ */
+#ifndef RGSIM_H
+#define RGSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RG_ABI_VERSION 1
+
+enum {
+    RG_OK = 0,
+    RG_ERR_ARG = 1,        /* invalid argument (null pointer, n, eps2 <= 0, ...) */
+    RG_ERR_HIP = 2,        /* HIP runtime error */
+    RG_ERR_NONFINITE = 3,  /* non-finite input detected */
+    RG_ERR_CAPACITY = 4,   /* buffer or size limit exceeded */
+    RG_ERR_ALIAS = 5       /* output aliases an input */
+};
+
+enum { RG_FP64 = 0, RG_FP32 = 1, RG_FP32C = 2 };
+
+/* Library ABI version (RG_ABI_VERSION). */
+int rg_version(void);
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char *rg_last_error(void);
+
+/* Launch-shape tunables (no reference counterpart; used by the autotuner):
+ *   "direct_ipt"  i-particles per lane in the direct kernels: 0 (heuristic),
+ *                 1, 2 or 4
+ *   "walk_block"  threads per block of the BH walk: 64, 128, 256 or 512
+ * Process-wide; takes effect at the next launch. */
+int rg_set_param(const char *name, int64_t value);
+
+/* ------------------------------------------------------------- direct N^2 */
+
+/* Replaces gravity.compute_acc_direct(pos, mass, acc, n, grav, eps2)
+ * (core/solvers/gravity.py:13-26).  acc[i] = G sum_j m_j d_ij / (|d_ij|^2 +
+ * eps2)^{3/2} over ALL j (j == i adds zero).  Overwrites acc.  eps2 must be
+ * > 0 (the reference yields NaN for eps2 == 0).  acc must not alias pos or
+ * mass. */
+int rg_direct_acc(const double *pos, const double *mass, double *acc, int64_t n,
+                  double G, double eps2, int precision, void *stream);
+
+/* rg_direct_acc followed by leapfrog.kick(vel, acc, n, dt_half)
+ * (core/integrators/leapfrog.py:11-15) fused into the force kernel's
+ * epilogue: the closing half-kick of Engine.step (core/engine.py:113-114). */
+int rg_direct_acc_kick(const double *pos, const double *mass, double *acc,
+                       double *vel, int64_t n, double G, double eps2,
+                       int precision, double dt_half, void *stream);
+
+/* ------------------------------------------------------------ integrator */
+
+/* leapfrog.kick (core/integrators/leapfrog.py:11-15): vel += acc * dt_half. */
+int rg_kick(double *vel, const double *acc, int64_t n, double dt_half, void *stream);
+
+/* leapfrog.drift (core/integrators/leapfrog.py:18-22): pos += vel * dt. */
+int rg_drift(double *pos, const double *vel, int64_t n, double dt, void *stream);
+
+/* kick(dt_half) then drift(dt) in one pass, bit-identical to the two calls
+ * in sequence (core/engine.py:111-112). */
+int rg_kick_drift(double *pos, double *vel, const double *acc, int64_t n,
+                  double dt_half, double dt, void *stream);
+
+/* ----------------------------------------------------------- diagnostics */
+
+/* Engine.energies (core/engine.py:119-124) -> gravity.kinetic_energy and
+ * gravity.potential_energy (core/solvers/gravity.py:29-57), plus total
+ * momentum.  out5 (device, 5 doubles) = {KE, PE, Px, Py, Pz}, all FP64,
+ * reduced in a fixed order (bitwise reproducible).  PE excludes j == i.
+ * pos == NULL skips the N^2 potential pass (PE = 0). */
+int rg_energy(const double *pos, const double *vel, const double *mass, int64_t n,
+              double G, double eps2, double *out5, void *stream);
+
+/* ------------------------------------------------------------ Barnes-Hut */
+
+/* barnes_hut.morton_codes(pos, bits) (core/solvers/barnes_hut.py:38-46):
+ * keys over the bounding cube with bits per axis (21 -> 63-bit keys,
+ * 10 -> 30-bit keys).  A NaN coordinate quantises to 0. */
+int rg_morton_codes(const double *pos, int64_t n, int bits, uint64_t *keys,
+                    void *stream);
+
+typedef struct rg_bh rg_bh;
+
+/* BarnesHut(n, theta, softening) (core/solvers/barnes_hut.py:223-240).
+ * n >= 2.  G is explicit (the reference hard-codes core.units.G).
+ * precision RG_FP64 walks FP64 node records with the reference's exact
+ * criterion and operation order; RG_FP32/RG_FP32C walk FP32 node records
+ * (aggregation stays FP64).  Allocates all scratch on the current device. */
+int rg_bh_create(int64_t n, double theta, double eps, double G, int morton_bits,
+                 int precision, rg_bh **out);
+
+/* BarnesHut.compute(pos, mass, acc) (core/solvers/barnes_hut.py:242-259):
+ * Morton keys, stable radix sort, Karras build, bottom-up aggregation and
+ * the theta walk, all on the device.  Overwrites acc. */
+int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc,
+                  void *stream);
+
+/* Copies of the last compute's tree in the reference's encoding (device
+ * pointers): keys_sorted (n) = code (barnes_hut.py:248), order (n),
+ * left/right (n-1), parent (2n-1; internal ids offset by n, root -1).
+ * Any pointer may be NULL. */
+int rg_bh_export_tree(rg_bh *h, uint64_t *keys_sorted, int32_t *order, int32_t *left,
+                      int32_t *right, int32_t *parent, void *stream);
+
+/* FP64 node data of the last compute, indexed by node id (2n-1):
+ * com (x3), nmass, nmin (x3), nmax (x3)  (barnes_hut.py:232-236).
+ * Any pointer may be NULL. */
+int rg_bh_export_nodes(rg_bh *h, double *com, double *nmass, double *nmin,
+                       double *nmax, void *stream);
+
+typedef struct {
+    int64_t n;
+    int64_t visits_total;   /* nodes examined by the walk, all particles */
+    int64_t visits_max;     /* max nodes examined by one particle */
+    int64_t nonfinite;      /* non-finite input coordinates seen */
+} rg_bh_stats_t;
+
+/* Statistics of the last compute.  Synchronises that compute's stream. */
+int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out);
+
+/* Per-phase timing with HIP events recorded on the compute's stream (no
+ * reference counterpart; the reference measures wall time only).  While
+ * enabled, each rg_bh_compute records 4 events (up to 4096 computes).
+ * rg_bh_timing waits for the recorded computes and returns the summed
+ * milliseconds of {bbox+Morton+sort, build+aggregation, walk} and how many
+ * computes they cover, then resets the count. */
+int rg_bh_set_timing(rg_bh *h, int enable);
+int rg_bh_timing(rg_bh *h, double *ms3, int64_t *count);
+
+int rg_bh_destroy(rg_bh *h);
+
+/* --------------------------------------------------------------- adjoint */
+
+/* Vector-Jacobian product of the direct acceleration (no reference
+ * counterpart; the reference has no gradients).  With d = x_j - x_k,
+ * s2 = |d|^2 + eps2 and T = I/s^3 - 3 d d^T / s^5:
+ *   g_pos[k]  = G sum_j T_kj (m_k lam_j - m_j lam_k)
+ *   g_mass[k] = -G sum_j lam_j . d / s^3
+ * Overwrites g_pos (n,3) and g_mass (n).  precision RG_FP64 or RG_FP32C. */
+int rg_direct_vjp(const double *pos, const double *mass, const double *lam,
+                  double *g_pos, double *g_mass, int64_t n, double G, double eps2,
+                  int precision, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RGSIM_H */

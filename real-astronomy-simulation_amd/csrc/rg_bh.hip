@@ -1,0 +1,932 @@
+// LBVH Barnes-Hut treecode, entirely on the device.
+//
+// Reference: core/solvers/barnes_hut.py (paths relative to the reference
+// checkout).  The reference computes Morton keys and a stable argsort on the
+// host in numpy, then builds a Karras radix tree, aggregates every node by
+// per-ancestor FP64 atomics and walks a per-particle stack.  Here:
+//
+//   1. k_bbox_*    FP64 bounding box (exact min/max, fixed-order reduction)
+//   2. k_morton    keys bit-identical to morton_codes (barnes_hut.py:38-46)
+//   3. k_radix_*   stable LSD radix sort of (key, index), 8-bit digits; the
+//                  permutation equals np.argsort(kind="stable") (:247)
+//   4. k_build     Karras build with the reference's exact delta (incl. the
+//                  index tie-break) using hardware clz (:50-121)
+//   5. k_aggregate leaf init + deterministic bottom-up aggregation: the
+//                  second child to finish combines both children (acquire /
+//                  release fences around an arrival counter), so every node
+//                  is computed exactly once and bitwise reproducibly (:124-177)
+//   6. k_walk      stackless right-first DFS with escape pointers: the same
+//                  node sequence the reference's stack walk pops (:180-217),
+//                  over node records laid out for one 32-byte (FP32) or
+//                  48-byte (FP64) fetch per visit.
+//
+// Node ids follow the reference: leaves 0..n-1 in sorted order, internal
+// node i has id i+n, root id n.
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "rg_internal.h"
+
+namespace {
+
+constexpr int kSortBS = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBS * kSortItems;  // keys per sort block
+constexpr int kBoxBS = 256;
+constexpr int kWalkBSDefault = 256;
+
+int g_walk_bs = kWalkBSDefault;
+
+struct alignas(16) WalkRec32 {
+    float x, y, z, gm;   // centre of mass, G * mass
+    float crit;          // accept when crit < r2 (leaves: -inf)
+    int right;           // right child id (leaves: -1)
+    int escape;          // next node in right-first DFS order after this subtree (-1: done)
+    int pad;
+};
+
+struct alignas(16) WalkRec64 {
+    double x, y, z, m;   // centre of mass, mass
+    double size;         // max-axis AABB extent
+    int right;
+    int escape;
+};
+
+// --------------------------------------------------------------- bbox
+__global__ __launch_bounds__(kBoxBS) void k_bbox_partial(const double *__restrict__ pos, int n,
+                                                          double *__restrict__ part)
+{
+    __shared__ double smin[3][kBoxBS];
+    __shared__ double smax[3][kBoxBS];
+    __shared__ double sbad[kBoxBS];
+    double lo[3] = {INFINITY, INFINITY, INFINITY};
+    double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double bad = 0.0;
+    for (int i = blockIdx.x * kBoxBS + threadIdx.x; i < n; i += gridDim.x * kBoxBS) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double v = pos[3 * i + a];
+            if (isfinite(v)) {
+                lo[a] = fmin(lo[a], v);
+                hi[a] = fmax(hi[a], v);
+            } else {
+                bad += 1.0;
+            }
+        }
+    }
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        smin[a][t] = lo[a];
+        smax[a][t] = hi[a];
+    }
+    sbad[t] = bad;
+    __syncthreads();
+    for (int w = kBoxBS / 2; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                smin[a][t] = fmin(smin[a][t], smin[a][t + w]);
+                smax[a][t] = fmax(smax[a][t], smax[a][t + w]);
+            }
+            sbad[t] += sbad[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int nb = gridDim.x;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            part[a * nb + blockIdx.x] = smin[a][0];
+            part[(3 + a) * nb + blockIdx.x] = smax[a][0];
+        }
+        part[6 * nb + blockIdx.x] = sbad[0];
+    }
+}
+
+// box = {lo_x, lo_y, lo_z, scale}; stats[0..1] reset, stats[2] = non-finite.
+__global__ __launch_bounds__(kBoxBS) void k_bbox_final(const double *__restrict__ part, int nb,
+                                                        int bits, double *__restrict__ box,
+                                                        unsigned long long *__restrict__ stats)
+{
+    __shared__ double s[7][kBoxBS];
+    const int t = threadIdx.x;
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int b = t; b < nb; b += kBoxBS) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            v[a] = fmin(v[a], part[a * nb + b]);
+            v[3 + a] = fmax(v[3 + a], part[(3 + a) * nb + b]);
+        }
+        v[6] += part[6 * nb + b];
+    }
+#pragma unroll
+    for (int c = 0; c < 7; ++c) s[c][t] = v[c];
+    __syncthreads();
+    for (int w = kBoxBS / 2; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                s[a][t] = fmin(s[a][t], s[a][t + w]);
+                s[3 + a][t] = fmax(s[3 + a][t], s[3 + a][t + w]);
+            }
+            s[6][t] += s[6][t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        // barnes_hut.py:40-43: span = max over axes of (max - lo), 1.0 if not
+        // positive; scale = (2^bits - 1) / (span * 1.0000001).
+        double span = s[3][0] - s[0][0];
+        for (int a = 1; a < 3; ++a) {
+            const double sa = s[3 + a][0] - s[a][0];
+            if (sa > span) span = sa;
+        }
+        if (!(span > 0.0)) span = 1.0;
+        const double qmax = static_cast<double>((1ull << bits) - 1ull);
+        box[0] = s[0][0];
+        box[1] = s[1][0];
+        box[2] = s[2][0];
+        box[3] = qmax / (span * 1.0000001);
+        if (stats) {
+            stats[0] = 0ull;
+            stats[1] = 0ull;
+            stats[2] = static_cast<unsigned long long>(s[6][0]);
+            stats[3] = 0ull;
+        }
+    }
+}
+
+// ------------------------------------------------------------- morton
+__device__ __forceinline__ uint64_t part1by2(uint64_t x)
+{
+    // barnes_hut.py:27-35: spread the low 21 bits to every third bit.
+    x &= 0x1FFFFFull;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ void k_morton(const double *__restrict__ pos, int n, int bits,
+                         const double *__restrict__ box, uint64_t *__restrict__ keys)
+{
+    const double lo0 = box[0], lo1 = box[1], lo2 = box[2], scale = box[3];
+    const double qmax = static_cast<double>((1ull << bits) - 1ull);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double lo[3] = {lo0, lo1, lo2};
+        uint64_t key = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            // (pos - lo) * scale, clipped to [0, qmax], truncated (numpy astype).
+            double t = (pos[3 * i + a] - lo[a]) * scale;
+            if (!(t >= 0.0)) t = 0.0;
+            if (t > qmax) t = qmax;
+            key |= part1by2(static_cast<uint64_t>(t)) << a;
+        }
+        keys[i] = key;
+    }
+}
+
+int launch_bbox_morton(const double *pos, int n, int bits, double *part, double *box,
+                       unsigned long long *stats, uint64_t *keys, hipStream_t st)
+{
+    const int nb = min((n + kBoxBS - 1) / kBoxBS, 1024);
+    k_bbox_partial<<<nb, kBoxBS, 0, st>>>(pos, n, part);
+    RG_CHECK_LAUNCH();
+    k_bbox_final<<<1, kBoxBS, 0, st>>>(part, nb, bits, box, stats);
+    RG_CHECK_LAUNCH();
+    const int mg = min((n + 255) / 256, 8192);
+    k_morton<<<mg, 256, 0, st>>>(pos, n, bits, box, keys);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
+size_t bbox_part_doubles(int n)
+{
+    const int nb = min((n + kBoxBS - 1) / kBoxBS, 1024);
+    return 7 * static_cast<size_t>(nb) + 8;
+}
+
+// --------------------------------------------------------- radix sort
+// Pass: hist -> exclusive scan (digit-major, block-minor) -> stable scatter.
+__global__ __launch_bounds__(kSortBS) void k_radix_hist(const uint64_t *__restrict__ keys, int n,
+                                                         int shift, int nblk,
+                                                         unsigned *__restrict__ hist)
+{
+    __shared__ unsigned s_h[256];
+    const int t = threadIdx.x;
+    s_h[t] = 0u;
+    __syncthreads();
+    const int base = blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortItems; ++r) {
+        const int i = base + r * kSortBS + t;
+        if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[t * nblk + blockIdx.x] = s_h[t];
+}
+
+constexpr int kScanBS = 1024;
+
+__global__ __launch_bounds__(kScanBS) void k_scan_exclusive(unsigned *__restrict__ a, int m)
+{
+    __shared__ unsigned s[kScanBS];
+    const int t = threadIdx.x;
+    const int per = (m + kScanBS - 1) / kScanBS;
+    const int b0 = t * per;
+    const int b1 = min(b0 + per, m);
+    unsigned sum = 0;
+    for (int k = b0; k < b1; ++k) sum += a[k];
+    s[t] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the per-thread sums.
+    for (int off = 1; off < kScanBS; off <<= 1) {
+        const unsigned v = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    unsigned run = s[t] - sum;
+    for (int k = b0; k < b1; ++k) {
+        const unsigned v = a[k];
+        a[k] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortBS) void k_radix_scatter(
+    const uint64_t *__restrict__ kin, const int *__restrict__ iin, uint64_t *__restrict__ kout,
+    int *__restrict__ iout, int n, int shift, int nblk, const unsigned *__restrict__ offs)
+{
+    constexpr int kWaves = kSortBS / 64;
+    constexpr int kRounds = kSortItems;  // each wave: kRounds rounds of 64 keys
+    __shared__ unsigned s_base[kWaves][256];
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    const int lane = t & 63;
+    for (int k = t; k < kWaves * 256; k += kSortBS) (&s_base[0][0])[k] = 0u;
+    __syncthreads();
+
+    const int seg = blockIdx.x * kSortTile + w * (kRounds * 64);
+    uint64_t key[kRounds];
+    int idv[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = seg + r * 64 + lane;
+        if (i < n) {
+            key[r] = kin[i];
+            idv[r] = iin ? iin[i] : i;
+            atomicAdd(&s_base[w][(key[r] >> shift) & 0xFFu], 1u);
+        } else {
+            key[r] = 0;
+            idv[r] = -1;
+        }
+    }
+    __syncthreads();
+    {
+        // Thread t owns digit t: wave-ordered bases from the global offset.
+        unsigned run = offs[t * nblk + blockIdx.x];
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww) {
+            const unsigned c = s_base[ww][t];
+            s_base[ww][t] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = seg + r * 64 + lane;
+        const bool valid = i < n;
+        const unsigned d = static_cast<unsigned>((key[r] >> shift) & 0xFFu);
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long bb = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        const unsigned rank = __popcll(peers & lt_mask);
+        const unsigned base = s_base[w][d];
+        if (valid) {
+            kout[base + rank] = key[r];
+            iout[base + rank] = idv[r];
+            if (rank == 0) s_base[w][d] = base + static_cast<unsigned>(__popcll(peers));
+        }
+    }
+}
+
+// ------------------------------------------------------------- build
+__device__ __forceinline__ int delta_dev(int64_t i, int64_t j, int64_t n, uint64_t ci,
+                                         const uint64_t *__restrict__ code)
+{
+    // barnes_hut.py:71-81.  _clz64 counts from bit 62 (= clzll - 1 for a
+    // 63-bit value); the equal-key tie-break 63 + _clz_i32(i ^ j) counts
+    // from bit 30 (= 62 + clz32).
+    if (j < 0 || j >= n) return -1;
+    const uint64_t x = ci ^ code[j];
+    if (x == 0) return 62 + __clz(static_cast<int>(i ^ j));
+    return __clzll(static_cast<long long>(x)) - 1;
+}
+
+__global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restrict__ left,
+                        int *__restrict__ right, int *__restrict__ parent,
+                        int *__restrict__ lo_range, int *__restrict__ left_at_split,
+                        int *__restrict__ arrive)
+{
+    const int64_t nn = n;
+    for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < n - 1;
+         ii += gridDim.x * blockDim.x) {
+        const int64_t i = ii;
+        const uint64_t ci = code[i];
+        // barnes_hut.py:92-121, step for step.
+        const int64_t d = delta_dev(i, i + 1, nn, ci, code) > delta_dev(i, i - 1, nn, ci, code) ? 1 : -1;
+        const int d_min = delta_dev(i, i - d, nn, ci, code);
+        int64_t l_max = 2;
+        while (delta_dev(i, i + l_max * d, nn, ci, code) > d_min) l_max *= 2;
+        int64_t l = 0;
+        for (int64_t t = l_max / 2; t >= 1; t /= 2)
+            if (delta_dev(i, i + (l + t) * d, nn, ci, code) > d_min) l += t;
+        const int64_t j = i + l * d;
+        const int d_node = delta_dev(i, j, nn, ci, code);
+        int64_t s = 0;
+        int64_t t2 = (l + 1) / 2;
+        while (t2 >= 1) {
+            if (delta_dev(i, i + (s + t2) * d, nn, ci, code) > d_node) s += t2;
+            t2 = t2 > 1 ? (t2 + 1) / 2 : 0;
+        }
+        const int64_t gamma = i + s * d + (d < 0 ? d : 0);
+        const int64_t lo = i < j ? i : j;
+        const int64_t hi = i < j ? j : i;
+        const int lc = static_cast<int>(lo == gamma ? gamma : gamma + nn);
+        const int rc = static_cast<int>(hi == gamma + 1 ? gamma + 1 : gamma + 1 + nn);
+        left[i] = lc;
+        right[i] = rc;
+        parent[lc] = static_cast<int>(i + nn);
+        parent[rc] = static_cast<int>(i + nn);
+        lo_range[i] = static_cast<int>(lo);
+        left_at_split[gamma] = lc;
+        arrive[i] = 0;
+        if (i == 0) parent[nn] = -1;
+    }
+}
+
+// --------------------------------------------------------- aggregation
+struct NodeArrays {
+    double *com, *nmass, *nmin, *nmax, *nmom;
+};
+
+template <int MODE>  // 0: FP64 walk records, 1: FP32 walk records
+__global__ void k_aggregate(const int *__restrict__ order, const double *__restrict__ pos,
+                            const double *__restrict__ mass, int n,
+                            const int *__restrict__ left, const int *__restrict__ right,
+                            const int *__restrict__ parent, const int *__restrict__ lo_range,
+                            const int *__restrict__ left_at_split, int *__restrict__ arrive,
+                            NodeArrays na, void *__restrict__ recs, double G, double inv_theta2,
+                            int theta_zero)
+{
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n;
+         li += gridDim.x * blockDim.x) {
+        const int p = order[li];
+        const double x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+        const double m = mass[p];
+        // barnes_hut.py:129-134 (leaf init).
+        na.com[3 * li] = x;
+        na.com[3 * li + 1] = y;
+        na.com[3 * li + 2] = z;
+        na.nmin[3 * li] = x;
+        na.nmin[3 * li + 1] = y;
+        na.nmin[3 * li + 2] = z;
+        na.nmax[3 * li] = x;
+        na.nmax[3 * li + 1] = y;
+        na.nmax[3 * li + 2] = z;
+        na.nmass[li] = m;
+        na.nmom[3 * li] = m * x;
+        na.nmom[3 * li + 1] = m * y;
+        na.nmom[3 * li + 2] = m * z;
+        const int esc_leaf = li == 0 ? -1 : left_at_split[li - 1];
+        if (MODE == 0) {
+            WalkRec64 r;
+            r.x = x; r.y = y; r.z = z; r.m = m;
+            r.size = 0.0;
+            r.right = -1;
+            r.escape = esc_leaf;
+            static_cast<WalkRec64 *>(recs)[li] = r;
+        } else {
+            WalkRec32 r;
+            r.x = static_cast<float>(x);
+            r.y = static_cast<float>(y);
+            r.z = static_cast<float>(z);
+            r.gm = static_cast<float>(G * m);
+            r.crit = -INFINITY;
+            r.right = -1;
+            r.escape = esc_leaf;
+            r.pad = 0;
+            static_cast<WalkRec32 *>(recs)[li] = r;
+        }
+        __threadfence();  // release the leaf before signalling its parent
+
+        int node = parent[li];
+        while (node >= 0) {
+            const int ii = node - n;
+            const int old = atomicAdd(&arrive[ii], 1);
+            if (old == 0) break;  // the sibling subtree is not finished; it will continue
+            __threadfence();      // acquire: the sibling's stores are visible
+            const int L = left[ii], R = right[ii];
+            const double mL = na.nmass[L], mR = na.nmass[R];
+            const double mm = mL + mR;
+            double mom[3], mn[3], mx[3], c[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                mom[a] = na.nmom[3 * L + a] + na.nmom[3 * R + a];
+                mn[a] = fmin(na.nmin[3 * L + a], na.nmin[3 * R + a]);
+                mx[a] = fmax(na.nmax[3 * L + a], na.nmax[3 * R + a]);
+                // barnes_hut.py:175-177: com = mom / mass where mass > 0.
+                c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
+                na.nmom[3 * node + a] = mom[a];
+                na.nmin[3 * node + a] = mn[a];
+                na.nmax[3 * node + a] = mx[a];
+                na.com[3 * node + a] = c[a];
+            }
+            na.nmass[node] = mm;
+            double size = mx[0] - mn[0];
+            const double sy = mx[1] - mn[1];
+            const double sz = mx[2] - mn[2];
+            size = sy > size ? sy : size;
+            size = sz > size ? sz : size;
+            const int lo = lo_range[ii];
+            const int esc = lo == 0 ? -1 : left_at_split[lo - 1];
+            if (MODE == 0) {
+                WalkRec64 r;
+                r.x = c[0]; r.y = c[1]; r.z = c[2]; r.m = mm;
+                r.size = size;
+                r.right = R;
+                r.escape = esc;
+                static_cast<WalkRec64 *>(recs)[node] = r;
+            } else {
+                WalkRec32 r;
+                r.x = static_cast<float>(c[0]);
+                r.y = static_cast<float>(c[1]);
+                r.z = static_cast<float>(c[2]);
+                r.gm = static_cast<float>(G * mm);
+                r.crit = theta_zero ? INFINITY : static_cast<float>(size * size * inv_theta2);
+                r.right = R;
+                r.escape = esc;
+                r.pad = 0;
+                static_cast<WalkRec32 *>(recs)[node] = r;
+            }
+            __threadfence();  // release this node before signalling its parent
+            node = parent[node];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- walk
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    // Blocks are dispatched round-robin over the 8 XCDs.  Give each XCD a
+    // contiguous run of Morton-ordered particles so the nodes its blocks
+    // share stay in that XCD's L2.
+    constexpr int kXcd = 8;
+    const int q = nb / kXcd, r = nb % kXcd;
+    const int x = b % kXcd, k = b / kXcd;
+    return x * q + min(x, r) + k;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// FP64: the reference's criterion and arithmetic (barnes_hut.py:196-216).
+__global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restrict__ order,
+                         double *__restrict__ acc, int n, double G, double theta2, double eps2,
+                         unsigned long long *__restrict__ stats)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int t = b * blockDim.x + threadIdx.x;
+    const bool active = t < n;
+    int visits = 0;
+    if (active) {
+        const WalkRec64 self = rec[t];
+        const double px = self.x, py = self.y, pz = self.z;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        int node = n;  // root
+        const int max_it = 2 * n;
+        const int m = 2 * n - 1;
+        while (node >= 0 && node < m && visits < max_it) {
+            ++visits;
+            const WalkRec64 r = rec[node];
+            const double dx = r.x - px;
+            const double dy = r.y - py;
+            const double dz = r.z - pz;
+            const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+            // Leaves always interact (the self leaf adds an exact zero: d == 0).
+            if (node < n || r.size * r.size < theta2 * r2) {
+                const double inv = 1.0 / sqrt(r2);
+                const double s = G * r.m * inv * inv * inv;
+                ax += s * dx;
+                ay += s * dy;
+                az += s * dz;
+                node = r.escape;
+            } else {
+                node = r.right;
+            }
+        }
+        if (node >= 0) atomicAdd(&stats[3], 1ull);  // walk did not terminate in bounds
+        const int p = order[t];
+        acc[3 * p] = ax;
+        acc[3 * p + 1] = ay;
+        acc[3 * p + 2] = az;
+    }
+    const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
+    const int mx = wave_max_i32(visits);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], tot);
+        atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+    }
+}
+
+__global__ void k_walk32(const WalkRec32 *__restrict__ rec, const int *__restrict__ order,
+                         double *__restrict__ acc, int n, float eps2,
+                         unsigned long long *__restrict__ stats)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int t = b * blockDim.x + threadIdx.x;
+    const bool active = t < n;
+    int visits = 0;
+    if (active) {
+        const float4 self = *reinterpret_cast<const float4 *>(&rec[t]);
+        const float px = self.x, py = self.y, pz = self.z;
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        int node = n;
+        const int max_it = 2 * n;
+        const int m = 2 * n - 1;
+        while (node >= 0 && node < m && visits < max_it) {
+            ++visits;
+            const float4 c = *reinterpret_cast<const float4 *>(&rec[node]);
+            const int4 a = *(reinterpret_cast<const int4 *>(&rec[node]) + 1);
+            const float dx = c.x - px;
+            const float dy = c.y - py;
+            const float dz = c.z - pz;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            if (__int_as_float(a.x) < r2) {
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float s = c.w * (inv * inv) * inv;
+                ax = fmaf(s, dx, ax);
+                ay = fmaf(s, dy, ay);
+                az = fmaf(s, dz, az);
+                node = a.z;
+            } else {
+                node = a.y;
+            }
+        }
+        if (node >= 0) atomicAdd(&stats[3], 1ull);
+        const int p = order[t];
+        acc[3 * p] = static_cast<double>(ax);
+        acc[3 * p + 1] = static_cast<double>(ay);
+        acc[3 * p + 2] = static_cast<double>(az);
+    }
+    const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
+    const int mx = wave_max_i32(visits);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], tot);
+        atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+    }
+}
+
+}  // namespace
+
+// ================================================================ handle
+struct rg_bh {
+    int64_t n = 0;
+    double theta = 0, eps = 0, G = 0;
+    int bits = 21;
+    int precision = RG_FP64;
+    int device = 0;
+    int nblk = 0;           // sort blocks
+    int passes = 0;         // radix passes
+    int final_buf = 0;      // ping-pong buffer holding the sorted result
+    hipStream_t last_stream = nullptr;
+    bool computed = false;
+
+    uint64_t *keys[2] = {nullptr, nullptr};
+    int *idx[2] = {nullptr, nullptr};
+    unsigned *hist = nullptr;
+    int *left = nullptr, *right = nullptr, *parent = nullptr;
+    int *lo_range = nullptr, *left_at_split = nullptr, *arrive = nullptr;
+    NodeArrays na{};
+    void *recs = nullptr;
+    double *box = nullptr;    // bbox partials followed by the 4-double box
+    unsigned long long *stats = nullptr;
+    std::vector<void *> allocs;
+
+    // Optional per-phase timing: 4 events per compute (start, sorted,
+    // aggregated, walked), up to kTimingCap computes between reads.
+    static constexpr int kTimingCap = 4096;
+    bool timing = false;
+    int timed = 0;
+    std::vector<hipEvent_t> ev;
+};
+
+namespace {
+
+int bh_alloc(rg_bh *h, void **p, size_t bytes)
+{
+    RG_HIP(hipMalloc(p, bytes > 0 ? bytes : 16));
+    h->allocs.push_back(*p);
+    return RG_OK;
+}
+
+void bh_free(rg_bh *h)
+{
+    for (void *p : h->allocs) (void)hipFree(p);
+    h->allocs.clear();
+    for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+    h->ev.clear();
+}
+
+int bh_mark(rg_bh *h, int phase, hipStream_t st)
+{
+    if (!h->timing || h->timed >= rg_bh::kTimingCap) return RG_OK;
+    RG_HIP(hipEventRecord(h->ev[4 * h->timed + phase], st));
+    return RG_OK;
+}
+
+}  // namespace
+
+int rg::rg_bh_set_walk_block_impl(int64_t v)
+{
+    RG_REQUIRE(v == 64 || v == 128 || v == 256 || v == 512, RG_ERR_ARG,
+               "walk_block must be 64, 128, 256 or 512");
+    g_walk_bs = static_cast<int>(v);
+    return RG_OK;
+}
+
+extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int morton_bits,
+                            int precision, rg_bh **out)
+{
+    RG_REQUIRE(out, RG_ERR_ARG, "null out");
+    *out = nullptr;
+    RG_REQUIRE(n >= 2 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "BH needs 2 <= n <= 2^29 (n=%lld)",
+               static_cast<long long>(n));
+    RG_REQUIRE(morton_bits >= 1 && morton_bits <= 21, RG_ERR_ARG, "morton_bits must be in [1, 21]");
+    RG_REQUIRE(theta >= 0.0 && std::isfinite(theta), RG_ERR_ARG, "theta must be finite and >= 0");
+    RG_REQUIRE(eps > 0.0 && std::isfinite(eps), RG_ERR_ARG, "softening must be > 0");
+    RG_REQUIRE(precision == RG_FP64 || precision == RG_FP32 || precision == RG_FP32C, RG_ERR_ARG,
+               "unknown precision %d", precision);
+    rg_bh *h = new (std::nothrow) rg_bh();
+    RG_REQUIRE(h, RG_ERR_CAPACITY, "out of host memory");
+    h->n = n;
+    h->theta = theta;
+    h->eps = eps;
+    h->G = G;
+    h->bits = morton_bits;
+    h->precision = precision;
+    hipError_t e = hipGetDevice(&h->device);
+    if (e != hipSuccess) {
+        delete h;
+        return rg::hip_fail(e, "hipGetDevice", __FILE__, __LINE__);
+    }
+    const size_t N = static_cast<size_t>(n);
+    const size_t M = 2 * N - 1;
+    h->nblk = static_cast<int>((N + kSortTile - 1) / kSortTile);
+    h->passes = (3 * morton_bits + 7) / 8;
+    int st = RG_OK;
+#define ALLOC(ptr, bytes)                                                  \
+    do {                                                                   \
+        void *p_ = nullptr;                                                \
+        st = bh_alloc(h, &p_, (bytes));                                    \
+        if (st != RG_OK) {                                                 \
+            bh_free(h);                                                    \
+            delete h;                                                      \
+            return st;                                                     \
+        }                                                                  \
+        (ptr) = reinterpret_cast<decltype(ptr)>(p_);                       \
+    } while (0)
+    ALLOC(h->keys[0], 8 * N);
+    ALLOC(h->keys[1], 8 * N);
+    ALLOC(h->idx[0], 4 * N);
+    ALLOC(h->idx[1], 4 * N);
+    ALLOC(h->hist, 4 * 256 * static_cast<size_t>(h->nblk));
+    ALLOC(h->left, 4 * (N - 1));
+    ALLOC(h->right, 4 * (N - 1));
+    ALLOC(h->lo_range, 4 * (N - 1));
+    ALLOC(h->left_at_split, 4 * (N - 1));
+    ALLOC(h->arrive, 4 * (N - 1));
+    ALLOC(h->parent, 4 * M);
+    ALLOC(h->na.com, 24 * M);
+    ALLOC(h->na.nmass, 8 * M);
+    ALLOC(h->na.nmin, 24 * M);
+    ALLOC(h->na.nmax, 24 * M);
+    ALLOC(h->na.nmom, 24 * M);
+    ALLOC(h->recs, (precision == RG_FP64 ? sizeof(WalkRec64) : sizeof(WalkRec32)) * M);
+    ALLOC(h->box, 8 * bbox_part_doubles(static_cast<int>(n)));
+    ALLOC(h->stats, 8 * 4);
+#undef ALLOC
+    *out = h;
+    return RG_OK;
+}
+
+extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc,
+                             void *stream)
+{
+    RG_REQUIRE(h, RG_ERR_ARG, "null handle");
+    RG_REQUIRE(pos && mass && acc, RG_ERR_ARG, "null pointer argument");
+    const int n = static_cast<int>(h->n);
+    const size_t b3 = 24 * static_cast<size_t>(n), b1 = 8 * static_cast<size_t>(n);
+    RG_REQUIRE(!rg_overlap(acc, b3, pos, b3) && !rg_overlap(acc, b3, mass, b1), RG_ERR_ALIAS,
+               "acc aliases pos or mass");
+    int dev = 0;
+    RG_HIP(hipGetDevice(&dev));
+    RG_REQUIRE(dev == h->device, RG_ERR_ARG, "handle was created on device %d, current is %d",
+               h->device, dev);
+    hipStream_t st = rg::as_stream(stream);
+
+    double *part = h->box + 8;
+    double *box = h->box;
+    int rc = bh_mark(h, 0, st);
+    if (rc != RG_OK) return rc;
+    rc = launch_bbox_morton(pos, n, h->bits, part, box, h->stats, h->keys[0], st);
+    if (rc != RG_OK) return rc;
+
+    // Stable LSD radix sort, 8-bit digits.
+    int cur = 0;
+    for (int p = 0; p < h->passes; ++p) {
+        const int shift = 8 * p;
+        k_radix_hist<<<h->nblk, kSortBS, 0, st>>>(h->keys[cur], n, shift, h->nblk, h->hist);
+        RG_CHECK_LAUNCH();
+        k_scan_exclusive<<<1, kScanBS, 0, st>>>(h->hist, 256 * h->nblk);
+        RG_CHECK_LAUNCH();
+        k_radix_scatter<<<h->nblk, kSortBS, 0, st>>>(h->keys[cur], p == 0 ? nullptr : h->idx[cur],
+                                                     h->keys[cur ^ 1], h->idx[cur ^ 1], n, shift,
+                                                     h->nblk, h->hist);
+        RG_CHECK_LAUNCH();
+        cur ^= 1;
+    }
+    h->final_buf = cur;
+    rc = bh_mark(h, 1, st);
+    if (rc != RG_OK) return rc;
+    const uint64_t *code = h->keys[cur];
+    const int *order = h->idx[cur];
+
+    const int g1 = min((n + 255) / 256, 16384);
+    k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->lo_range,
+                                h->left_at_split, h->arrive);
+    RG_CHECK_LAUNCH();
+
+    const double theta2 = h->theta * h->theta;
+    const int theta_zero = !(theta2 > 0.0);
+    const double inv_theta2 = theta_zero ? 0.0 : 1.0 / theta2;
+    if (h->precision == RG_FP64) {
+        k_aggregate<0><<<g1, 256, 0, st>>>(order, pos, mass, n, h->left, h->right, h->parent,
+                                           h->lo_range, h->left_at_split, h->arrive, h->na,
+                                           h->recs, h->G, inv_theta2, theta_zero);
+    } else {
+        k_aggregate<1><<<g1, 256, 0, st>>>(order, pos, mass, n, h->left, h->right, h->parent,
+                                           h->lo_range, h->left_at_split, h->arrive, h->na,
+                                           h->recs, h->G, inv_theta2, theta_zero);
+    }
+    RG_CHECK_LAUNCH();
+    rc = bh_mark(h, 2, st);
+    if (rc != RG_OK) return rc;
+
+    const int wb = g_walk_bs;
+    const int wg = (n + wb - 1) / wb;
+    const double eps2 = h->eps * h->eps;
+    if (h->precision == RG_FP64) {
+        k_walk64<<<wg, wb, 0, st>>>(static_cast<const WalkRec64 *>(h->recs), order, acc, n, h->G,
+                                    theta2, eps2, h->stats);
+    } else {
+        k_walk32<<<wg, wb, 0, st>>>(static_cast<const WalkRec32 *>(h->recs), order, acc, n,
+                                    static_cast<float>(eps2), h->stats);
+    }
+    RG_CHECK_LAUNCH();
+    rc = bh_mark(h, 3, st);
+    if (rc != RG_OK) return rc;
+    if (h->timing && h->timed < rg_bh::kTimingCap) ++h->timed;
+    h->last_stream = st;
+    h->computed = true;
+    return RG_OK;
+}
+
+extern "C" int rg_bh_set_timing(rg_bh *h, int enable)
+{
+    RG_REQUIRE(h, RG_ERR_ARG, "null handle");
+    if (enable && h->ev.empty()) {
+        h->ev.resize(4 * rg_bh::kTimingCap, nullptr);
+        for (auto &e : h->ev) RG_HIP(hipEventCreate(&e));
+    }
+    h->timing = enable != 0;
+    h->timed = 0;
+    return RG_OK;
+}
+
+extern "C" int rg_bh_timing(rg_bh *h, double *ms3, int64_t *count)
+{
+    RG_REQUIRE(h && ms3 && count, RG_ERR_ARG, "null argument");
+    ms3[0] = ms3[1] = ms3[2] = 0.0;
+    *count = h->timed;
+    for (int k = 0; k < h->timed; ++k) {
+        hipEvent_t *e = &h->ev[4 * k];
+        RG_HIP(hipEventSynchronize(e[3]));
+        for (int ph = 0; ph < 3; ++ph) {
+            float ms = 0.f;
+            RG_HIP(hipEventElapsedTime(&ms, e[ph], e[ph + 1]));
+            ms3[ph] += ms;
+        }
+    }
+    h->timed = 0;
+    return RG_OK;
+}
+
+extern "C" int rg_bh_export_tree(rg_bh *h, uint64_t *keys_sorted, int32_t *order, int32_t *left,
+                                 int32_t *right, int32_t *parent, void *stream)
+{
+    RG_REQUIRE(h, RG_ERR_ARG, "null handle");
+    RG_REQUIRE(h->computed, RG_ERR_ARG, "rg_bh_compute has not run on this handle");
+    hipStream_t st = rg::as_stream(stream);
+    const size_t N = static_cast<size_t>(h->n);
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    if (keys_sorted) RG_HIP(hipMemcpyAsync(keys_sorted, h->keys[h->final_buf], 8 * N, k, st));
+    if (order) RG_HIP(hipMemcpyAsync(order, h->idx[h->final_buf], 4 * N, k, st));
+    if (left) RG_HIP(hipMemcpyAsync(left, h->left, 4 * (N - 1), k, st));
+    if (right) RG_HIP(hipMemcpyAsync(right, h->right, 4 * (N - 1), k, st));
+    if (parent) RG_HIP(hipMemcpyAsync(parent, h->parent, 4 * (2 * N - 1), k, st));
+    return RG_OK;
+}
+
+extern "C" int rg_bh_export_nodes(rg_bh *h, double *com, double *nmass, double *nmin,
+                                  double *nmax, void *stream)
+{
+    RG_REQUIRE(h, RG_ERR_ARG, "null handle");
+    RG_REQUIRE(h->computed, RG_ERR_ARG, "rg_bh_compute has not run on this handle");
+    hipStream_t st = rg::as_stream(stream);
+    const size_t M = 2 * static_cast<size_t>(h->n) - 1;
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    if (com) RG_HIP(hipMemcpyAsync(com, h->na.com, 24 * M, k, st));
+    if (nmass) RG_HIP(hipMemcpyAsync(nmass, h->na.nmass, 8 * M, k, st));
+    if (nmin) RG_HIP(hipMemcpyAsync(nmin, h->na.nmin, 24 * M, k, st));
+    if (nmax) RG_HIP(hipMemcpyAsync(nmax, h->na.nmax, 24 * M, k, st));
+    return RG_OK;
+}
+
+extern "C" int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out)
+{
+    RG_REQUIRE(h && out, RG_ERR_ARG, "null argument");
+    RG_REQUIRE(h->computed, RG_ERR_ARG, "rg_bh_compute has not run on this handle");
+    RG_HIP(hipStreamSynchronize(h->last_stream));
+    unsigned long long s[4] = {0, 0, 0, 0};
+    RG_HIP(hipMemcpy(s, h->stats, sizeof(s), hipMemcpyDeviceToHost));
+    out->n = h->n;
+    out->visits_total = static_cast<int64_t>(s[0]);
+    out->visits_max = static_cast<int64_t>(s[1]);
+    out->nonfinite = static_cast<int64_t>(s[2]);
+    RG_REQUIRE(s[3] == 0, RG_ERR_CAPACITY,
+               "walk exceeded 2n node visits for %llu particles (corrupt tree)", s[3]);
+    return RG_OK;
+}
+
+extern "C" int rg_bh_destroy(rg_bh *h)
+{
+    if (!h) return RG_OK;
+    int prev = 0;
+    RG_HIP(hipGetDevice(&prev));
+    if (prev != h->device) RG_HIP(hipSetDevice(h->device));
+    bh_free(h);
+    if (prev != h->device) RG_HIP(hipSetDevice(prev));
+    delete h;
+    return RG_OK;
+}
+
+extern "C" int rg_morton_codes(const double *pos, int64_t n, int bits, uint64_t *keys,
+                               void *stream)
+{
+    RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "n out of range");
+    if (n == 0) return RG_OK;
+    RG_REQUIRE(pos && keys, RG_ERR_ARG, "null pointer argument");
+    RG_REQUIRE(bits >= 1 && bits <= 21, RG_ERR_ARG, "bits must be in [1, 21]");
+    void *scratch = nullptr;
+    const int ni = static_cast<int>(n);
+    const int st = rg::device_scratch(8 * bbox_part_doubles(ni), &scratch);
+    if (st != RG_OK) return st;
+    double *box = static_cast<double *>(scratch);
+    return launch_bbox_morton(pos, ni, bits, box + 8, box, nullptr, keys, rg::as_stream(stream));
+}

@@ -1,0 +1,73 @@
+// Error reporting, version and the per-device scratch buffer.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "rg_internal.h"
+
+namespace {
+thread_local char g_err[512] = "";
+
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_scratch_mu;
+std::vector<Scratch> g_scratch;
+}  // namespace
+
+namespace rg {
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+    set_error("HIP error %d (%s) in %s at %s:%d", static_cast<int>(e), hipGetErrorString(e),
+              what, file, line);
+    return RG_ERR_HIP;
+}
+
+int device_scratch(size_t bytes, void **ptr)
+{
+    int dev = 0;
+    RG_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch.size() <= static_cast<size_t>(dev)) g_scratch.resize(dev + 1);
+    Scratch &s = g_scratch[dev];
+    if (s.bytes < bytes) {
+        // Grow geometrically.  hipFree waits for the device, so a kernel
+        // still reading the old buffer finishes first.
+        size_t nb = s.bytes ? s.bytes : 4096;
+        while (nb < bytes) nb *= 2;
+        if (s.ptr) RG_HIP(hipFree(s.ptr));
+        s.ptr = nullptr;
+        s.bytes = 0;
+        RG_HIP(hipMalloc(&s.ptr, nb));
+        s.bytes = nb;
+    }
+    *ptr = s.ptr;
+    return RG_OK;
+}
+
+}  // namespace rg
+
+extern "C" int rg_version(void) { return RG_ABI_VERSION; }
+
+extern "C" const char *rg_last_error(void) { return g_err; }
+
+extern "C" int rg_set_param(const char *name, int64_t value)
+{
+    RG_REQUIRE(name, RG_ERR_ARG, "null name");
+    if (std::strcmp(name, "direct_ipt") == 0) return rg::rg_direct_set_ipt_impl(value);
+    if (std::strcmp(name, "walk_block") == 0) return rg::rg_bh_set_walk_block_impl(value);
+    rg::set_error("unknown parameter '%s'", name);
+    return RG_ERR_ARG;
+}

@@ -1,0 +1,454 @@
+// Direct N^2 gravity, leapfrog kick/drift and FP64 energy diagnostics.
+//
+// Reference: core/solvers/gravity.py:13-57, core/integrators/leapfrog.py:11-22
+// (paths relative to the reference checkout).
+//
+// Layout: the caller's FP64 (n,3) positions and (n,) masses are read in
+// place.  Each block stages a tile of j-particles into LDS, converting on
+// the fly to the precision of the pair arithmetic, and every lane then
+// streams the tile from LDS (all lanes read the same address: a broadcast)
+// against the i-particles it holds in registers.  The pair sum is a
+// vector-ALU workload (one rsqrt, about 16 VALU slots per pair); it is not a
+// dense contraction, so there is no MFMA here.
+//
+// This file is compiled with -ffp-contract=off: every multiply-add that is
+// fused is written as fma()/fmaf().  The FP64 kernels keep the reference's
+// expression order with IEEE 1.0/sqrt, so each particle's FP64 result is the
+// same double the sequential-j CPU evaluation gives.
+#include <cfloat>
+
+#include "rg_internal.h"
+
+namespace {
+
+constexpr int kBS = 256;      // threads per block = j-tile length
+constexpr int kChunk = 32;    // FP32C: pair terms per partial before the two-sum fold
+
+int g_direct_ipt = 0;         // 0 = heuristic; else i-particles per thread (1, 2, 4)
+
+__device__ __forceinline__ void split_df(double x, float &hi, float &lo)
+{
+    hi = static_cast<float>(x);
+    lo = static_cast<float>(x - static_cast<double>(hi));
+}
+
+// Knuth's TwoSum: s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
+{
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// MODE 0: FP64, reference order.  MODE 1: FP32.  MODE 2: FP32 on hi/lo
+// positions with chunked two-sum compensation.
+template <int MODE, int IPT>
+__global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
+                                                const double *__restrict__ mass,
+                                                double *__restrict__ acc,
+                                                double *__restrict__ vel, int n, double G,
+                                                double eps2, double dt_half)
+{
+    using T4 = typename std::conditional<MODE == 0, double4, float4>::type;
+    __shared__ T4 s_hi[kBS];
+    __shared__ float4 s_lo[MODE == 2 ? kBS : 1];
+
+    const int tid = threadIdx.x;
+    const int base = blockIdx.x * (kBS * IPT) + tid;
+
+    double xd[IPT], yd[IPT], zd[IPT];             // MODE 0 state
+    double axd[IPT], ayd[IPT], azd[IPT];
+    float xh[IPT], yh[IPT], zh[IPT];              // MODE 1/2 state
+    float xl[IPT], yl[IPT], zl[IPT];
+    float sx[IPT], sy[IPT], sz[IPT];              // running sums
+    float cx[IPT], cy[IPT], cz[IPT];              // compensation terms
+
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int i = base + k * kBS;
+        const bool ok = i < n;
+        const double px = ok ? pos[3 * i] : 0.0;
+        const double py = ok ? pos[3 * i + 1] : 0.0;
+        const double pz = ok ? pos[3 * i + 2] : 0.0;
+        xd[k] = px; yd[k] = py; zd[k] = pz;
+        axd[k] = ayd[k] = azd[k] = 0.0;
+        split_df(px, xh[k], xl[k]);
+        split_df(py, yh[k], yl[k]);
+        split_df(pz, zh[k], zl[k]);
+        sx[k] = sy[k] = sz[k] = 0.f;
+        cx[k] = cy[k] = cz[k] = 0.f;
+    }
+    const float eps2f = static_cast<float>(eps2);
+
+    for (int t0 = 0; t0 < n; t0 += kBS) {
+        const int j = t0 + tid;
+        if (j < n) {
+            const double qx = pos[3 * j], qy = pos[3 * j + 1], qz = pos[3 * j + 2];
+            const double qm = mass[j];
+            if constexpr (MODE == 0) {
+                s_hi[tid] = make_double4(qx, qy, qz, qm);
+            } else if constexpr (MODE == 1) {
+                s_hi[tid] = make_float4(static_cast<float>(qx), static_cast<float>(qy),
+                                        static_cast<float>(qz), static_cast<float>(G * qm));
+            } else {
+                float hx, lx, hy, ly, hz, lz;
+                split_df(qx, hx, lx);
+                split_df(qy, hy, ly);
+                split_df(qz, hz, lz);
+                s_hi[tid] = make_float4(hx, hy, hz, static_cast<float>(G * qm));
+                s_lo[tid] = make_float4(lx, ly, lz, 0.f);
+            }
+        }
+        __syncthreads();
+        const int cnt = min(kBS, n - t0);
+        if constexpr (MODE == 0) {
+            for (int jj = 0; jj < cnt; ++jj) {
+                const double4 q = s_hi[jj];
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) {
+                    // gravity.py:22-25, same association, IEEE sqrt and divide.
+                    const double dx = q.x - xd[k];
+                    const double dy = q.y - yd[k];
+                    const double dz = q.z - zd[k];
+                    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                    const double inv = 1.0 / sqrt(r2);
+                    const double s = G * q.w * inv * inv * inv;
+                    axd[k] += s * dx;
+                    ayd[k] += s * dy;
+                    azd[k] += s * dz;
+                }
+            }
+        } else if constexpr (MODE == 1) {
+#pragma unroll 4
+            for (int jj = 0; jj < cnt; ++jj) {
+                const float4 q = s_hi[jj];
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) {
+                    const float dx = q.x - xh[k];
+                    const float dy = q.y - yh[k];
+                    const float dz = q.z - zh[k];
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float s = q.w * (inv * inv) * inv;
+                    sx[k] = fmaf(s, dx, sx[k]);
+                    sy[k] = fmaf(s, dy, sy[k]);
+                    sz[k] = fmaf(s, dz, sz[k]);
+                }
+            }
+        } else {
+            for (int c0 = 0; c0 < cnt; c0 += kChunk) {
+                const int c1 = min(c0 + kChunk, cnt);
+                float px[IPT], py[IPT], pz[IPT];
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
+#pragma unroll 4
+                for (int jj = c0; jj < c1; ++jj) {
+                    const float4 q = s_hi[jj];
+                    const float4 l = s_lo[jj];
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) {
+                        // hi difference is exact for nearby pairs (Sterbenz);
+                        // the lo difference restores the FP64 separation.
+                        const float dx = (q.x - xh[k]) + (l.x - xl[k]);
+                        const float dy = (q.y - yh[k]) + (l.y - yl[k]);
+                        const float dz = (q.z - zh[k]) + (l.z - zl[k]);
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        const float s = q.w * (inv * inv) * inv;
+                        px[k] = fmaf(s, dx, px[k]);
+                        py[k] = fmaf(s, dy, py[k]);
+                        pz[k] = fmaf(s, dz, pz[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) {
+                    float e;
+                    two_sum(sx[k], px[k], sx[k], e);
+                    cx[k] += e;
+                    two_sum(sy[k], py[k], sy[k], e);
+                    cy[k] += e;
+                    two_sum(sz[k], pz[k], sz[k], e);
+                    cz[k] += e;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int i = base + k * kBS;
+        if (i >= n) continue;
+        double ax, ay, az;
+        if constexpr (MODE == 0) {
+            ax = axd[k]; ay = ayd[k]; az = azd[k];
+        } else {
+            ax = static_cast<double>(sx[k]) + static_cast<double>(cx[k]);
+            ay = static_cast<double>(sy[k]) + static_cast<double>(cy[k]);
+            az = static_cast<double>(sz[k]) + static_cast<double>(cz[k]);
+        }
+        acc[3 * i] = ax;
+        acc[3 * i + 1] = ay;
+        acc[3 * i + 2] = az;
+        if (vel) {  // leapfrog.py:14, v += a * dt_half
+            vel[3 * i] = vel[3 * i] + ax * dt_half;
+            vel[3 * i + 1] = vel[3 * i + 1] + ay * dt_half;
+            vel[3 * i + 2] = vel[3 * i + 2] + az * dt_half;
+        }
+    }
+}
+
+template <int MODE>
+int launch_direct(const double *pos, const double *mass, double *acc, double *vel, int n,
+                  double G, double eps2, double dt_half, hipStream_t st)
+{
+    int ipt = g_direct_ipt;
+    if (ipt == 0) {
+        // Keep at least ~2 blocks per CU (256 CUs) in flight; otherwise give
+        // each lane more i-particles to amortise the LDS broadcast reads.
+        ipt = 1;
+        if (MODE != 0) {
+            if (n >= 256 * kBS * 4 * 2) ipt = 4;
+            else if (n >= 256 * kBS * 2 * 2) ipt = 2;
+        }
+    }
+    const int per_block = kBS * ipt;
+    const int grid = (n + per_block - 1) / per_block;
+    switch (ipt) {
+    case 1:
+        k_direct<MODE, 1><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half);
+        break;
+    case 2:
+        k_direct<MODE, 2><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half);
+        break;
+    case 4:
+        k_direct<MODE, 4><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half);
+        break;
+    default:
+        rg::set_error("direct_ipt must be 1, 2 or 4 (got %d)", ipt);
+        return RG_ERR_ARG;
+    }
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
+int direct_common(const double *pos, const double *mass, double *acc, double *vel,
+                  int64_t n, double G, double eps2, int precision, double dt_half,
+                  void *stream)
+{
+    RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY,
+               "n=%lld outside [0, 2^29]", static_cast<long long>(n));
+    if (n == 0) return RG_OK;
+    RG_REQUIRE(pos && mass && acc, RG_ERR_ARG, "null pointer argument");
+    RG_REQUIRE(eps2 > 0.0, RG_ERR_ARG, "eps2 must be > 0 (got %g)", eps2);
+    const size_t b3 = sizeof(double) * 3 * static_cast<size_t>(n);
+    const size_t b1 = sizeof(double) * static_cast<size_t>(n);
+    RG_REQUIRE(!rg_overlap(acc, b3, pos, b3) && !rg_overlap(acc, b3, mass, b1), RG_ERR_ALIAS,
+               "acc aliases pos or mass");
+    if (vel)
+        RG_REQUIRE(!rg_overlap(vel, b3, pos, b3) && !rg_overlap(vel, b3, mass, b1) &&
+                       !rg_overlap(vel, b3, acc, b3),
+                   RG_ERR_ALIAS, "vel aliases pos, mass or acc");
+    hipStream_t st = rg::as_stream(stream);
+    const int ni = static_cast<int>(n);
+    switch (precision) {
+    case RG_FP64: return launch_direct<0>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
+    case RG_FP32: return launch_direct<1>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
+    case RG_FP32C: return launch_direct<2>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
+    default: break;
+    }
+    rg::set_error("unknown precision %d", precision);
+    return RG_ERR_ARG;
+}
+
+// ---------------------------------------------------------------- leapfrog
+__global__ void k_kick(double *__restrict__ v, const double *__restrict__ a, int m, double h)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+        v[k] = v[k] + a[k] * h;
+}
+
+__global__ void k_drift(double *__restrict__ x, const double *__restrict__ v, int m, double dt)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+        x[k] = x[k] + v[k] * dt;
+}
+
+__global__ void k_kick_drift(double *__restrict__ x, double *__restrict__ v,
+                             const double *__restrict__ a, int m, double h, double dt)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        const double vk = v[k] + a[k] * h;
+        v[k] = vk;
+        x[k] = x[k] + vk * dt;
+    }
+}
+
+int elementwise_grid(int m)
+{
+    const int g = (m + 255) / 256;
+    return g < 4096 ? (g > 0 ? g : 1) : 4096;
+}
+
+// --------------------------------------------------------------- energies
+constexpr int kEB = 256;
+
+// Block-level fixed-order tree reduction of 5 doubles per thread.
+__device__ void block_reduce5(double v[5], double *out, int stride)
+{
+    __shared__ double red[5][kEB];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) red[c][t] = v[c];
+    __syncthreads();
+    for (int w = kEB / 2; w > 0; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) red[c][t] = red[c][t] + red[c][t + w];
+        __syncthreads();
+    }
+    if (t == 0)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) out[c * stride] = red[c][0];
+}
+
+// Per-particle KE (gravity.py:35), PE (gravity.py:47-56, j != i) and
+// momentum; one partial row per block.
+__global__ __launch_bounds__(kEB) void k_energy_partial(const double *__restrict__ pos,
+                                                         const double *__restrict__ vel,
+                                                         const double *__restrict__ mass,
+                                                         int n, double G, double eps2,
+                                                         double *__restrict__ part)
+{
+    __shared__ double4 sj[kEB];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * kEB + tid;
+    const bool ok = i < n;
+    const bool with_pe = pos != nullptr;  // uniform across the grid
+    const double px = ok && with_pe ? pos[3 * i] : 0.0;
+    const double py = ok && with_pe ? pos[3 * i + 1] : 0.0;
+    const double pz = ok && with_pe ? pos[3 * i + 2] : 0.0;
+    double acc_i = 0.0;
+    for (int t0 = 0; with_pe && t0 < n; t0 += kEB) {
+        const int j = t0 + tid;
+        if (j < n) sj[tid] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], mass[j]);
+        __syncthreads();
+        const int cnt = min(kEB, n - t0);
+        for (int jj = 0; jj < cnt; ++jj) {
+            const double4 q = sj[jj];
+            if (t0 + jj != i) {
+                const double dx = q.x - px;
+                const double dy = q.y - py;
+                const double dz = q.z - pz;
+                const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                acc_i += q.w / sqrt(r2);
+            }
+        }
+        __syncthreads();
+    }
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (ok) {
+        const double mi = mass[i];
+        const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        v[0] = 0.5 * mi * (vx * vx + vy * vy + vz * vz);
+        v[1] = -0.5 * G * mi * acc_i;
+        v[2] = mi * vx;
+        v[3] = mi * vy;
+        v[4] = mi * vz;
+    }
+    block_reduce5(v, part + blockIdx.x, gridDim.x);
+}
+
+// Single block: thread t sums partials t, t+kEB, ... in order, then a tree.
+__global__ __launch_bounds__(kEB) void k_energy_final(const double *__restrict__ part, int nb,
+                                                       double *__restrict__ out5)
+{
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += kEB)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) v[c] = v[c] + part[c * nb + b];
+    block_reduce5(v, out5, 1);
+}
+
+}  // namespace
+
+int rg::rg_direct_set_ipt_impl(int64_t v)
+{
+    RG_REQUIRE(v == 0 || v == 1 || v == 2 || v == 4, RG_ERR_ARG, "direct_ipt must be 0, 1, 2 or 4");
+    g_direct_ipt = static_cast<int>(v);
+    return RG_OK;
+}
+
+extern "C" int rg_direct_acc(const double *pos, const double *mass, double *acc, int64_t n,
+                             double G, double eps2, int precision, void *stream)
+{
+    return direct_common(pos, mass, acc, nullptr, n, G, eps2, precision, 0.0, stream);
+}
+
+extern "C" int rg_direct_acc_kick(const double *pos, const double *mass, double *acc,
+                                  double *vel, int64_t n, double G, double eps2,
+                                  int precision, double dt_half, void *stream)
+{
+    RG_REQUIRE(vel || n == 0, RG_ERR_ARG, "null vel");
+    return direct_common(pos, mass, acc, vel, n, G, eps2, precision, dt_half, stream);
+}
+
+extern "C" int rg_kick(double *vel, const double *acc, int64_t n, double dt_half, void *stream)
+{
+    RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "n out of range");
+    if (n == 0) return RG_OK;
+    RG_REQUIRE(vel && acc, RG_ERR_ARG, "null pointer argument");
+    const int m = static_cast<int>(3 * n);
+    k_kick<<<elementwise_grid(m), 256, 0, rg::as_stream(stream)>>>(vel, acc, m, dt_half);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
+extern "C" int rg_drift(double *pos, const double *vel, int64_t n, double dt, void *stream)
+{
+    RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "n out of range");
+    if (n == 0) return RG_OK;
+    RG_REQUIRE(pos && vel, RG_ERR_ARG, "null pointer argument");
+    const int m = static_cast<int>(3 * n);
+    k_drift<<<elementwise_grid(m), 256, 0, rg::as_stream(stream)>>>(pos, vel, m, dt);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
+extern "C" int rg_kick_drift(double *pos, double *vel, const double *acc, int64_t n,
+                             double dt_half, double dt, void *stream)
+{
+    RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "n out of range");
+    if (n == 0) return RG_OK;
+    RG_REQUIRE(pos && vel && acc, RG_ERR_ARG, "null pointer argument");
+    const size_t b3 = sizeof(double) * 3 * static_cast<size_t>(n);
+    RG_REQUIRE(!rg_overlap(pos, b3, vel, b3) && !rg_overlap(pos, b3, acc, b3) &&
+                   !rg_overlap(vel, b3, acc, b3),
+               RG_ERR_ALIAS, "pos, vel and acc must not overlap");
+    const int m = static_cast<int>(3 * n);
+    k_kick_drift<<<elementwise_grid(m), 256, 0, rg::as_stream(stream)>>>(pos, vel, acc, m,
+                                                                         dt_half, dt);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
+extern "C" int rg_energy(const double *pos, const double *vel, const double *mass, int64_t n,
+                         double G, double eps2, double *out5, void *stream)
+{
+    RG_REQUIRE(n >= 1 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "n=%lld outside [1, 2^29]",
+               static_cast<long long>(n));
+    RG_REQUIRE(vel && mass && out5, RG_ERR_ARG, "null pointer argument");
+    RG_REQUIRE(!pos || eps2 > 0.0, RG_ERR_ARG, "eps2 must be > 0 (got %g)", eps2);
+    const int ni = static_cast<int>(n);
+    const int nb = (ni + kEB - 1) / kEB;
+    void *scratch = nullptr;
+    const int st = rg::device_scratch(sizeof(double) * 5 * static_cast<size_t>(nb), &scratch);
+    if (st != RG_OK) return st;
+    double *part = static_cast<double *>(scratch);
+    hipStream_t s = rg::as_stream(stream);
+    k_energy_partial<<<nb, kEB, 0, s>>>(pos, vel, mass, ni, G, eps2, part);
+    RG_CHECK_LAUNCH();
+    k_energy_final<<<1, kEB, 0, s>>>(part, nb, out5);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}

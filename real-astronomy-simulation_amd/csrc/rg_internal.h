@@ -1,0 +1,53 @@
+// Internal helpers shared by the librgsim translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rgsim.h"
+
+namespace rg {
+
+// Records a printf-style message as this thread's last error.
+void set_error(const char *fmt, ...);
+
+// Records a HIP failure and returns RG_ERR_HIP.
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+// Per-device, grow-only library scratch (see the header's scratch note).
+// Returns RG_OK and sets *ptr, or an error status.
+int device_scratch(size_t bytes, void **ptr);
+
+// Tunables behind rg_set_param.
+int rg_direct_set_ipt_impl(int64_t v);
+int rg_bh_set_walk_block_impl(int64_t v);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace rg
+
+#define RG_HIP(expr)                                                      \
+    do {                                                                  \
+        hipError_t rg_e_ = (expr);                                        \
+        if (rg_e_ != hipSuccess)                                          \
+            return rg::hip_fail(rg_e_, #expr, __FILE__, __LINE__);        \
+    } while (0)
+
+#define RG_CHECK_LAUNCH() RG_HIP(hipGetLastError())
+
+#define RG_REQUIRE(cond, code, ...)                                       \
+    do {                                                                  \
+        if (!(cond)) {                                                    \
+            rg::set_error(__VA_ARGS__);                                   \
+            return (code);                                                \
+        }                                                                 \
+    } while (0)
+
+// Byte ranges [a, a+na) and [b, b+nb) overlap.
+inline bool rg_overlap(const void *a, size_t na, const void *b, size_t nb)
+{
+    const char *pa = static_cast<const char *>(a);
+    const char *pb = static_cast<const char *>(b);
+    return pa < pb + nb && pb < pa + na;
+}

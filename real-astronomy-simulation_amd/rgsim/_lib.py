@@ -1,0 +1,114 @@
+"""ctypes binding of librgsim.so (the HIP C ABI declared in include/rgsim.h).
+
+The library must exist (built in-tree by ``build.py``) and a GPU must be
+present for any compute call: there is no CPU fallback.  Errors from the ABI
+raise ``RuntimeError`` carrying ``rg_last_error()``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librgsim.so")
+
+RG_OK = 0
+RG_ERR_ARG = 1
+RG_ERR_HIP = 2
+RG_ERR_NONFINITE = 3
+RG_ERR_CAPACITY = 4
+RG_ERR_ALIAS = 5
+
+PRECISIONS = {"fp64": 0, "fp32": 1, "fp32c": 2}
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_i = ctypes.c_int
+
+
+class BHStats(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("visits_total", ctypes.c_int64),
+                ("visits_max", ctypes.c_int64), ("nonfinite", ctypes.c_int64)]
+
+
+# name -> (restype, argtypes); every symbol include/rgsim.h declares.
+SIGNATURES = {
+    "rg_version": (_i, []),
+    "rg_last_error": (ctypes.c_char_p, []),
+    "rg_set_param": (_i, [ctypes.c_char_p, _i64]),
+    "rg_direct_acc": (_i, [_vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
+    "rg_direct_acc_kick": (_i, [_vp, _vp, _vp, _vp, _i64, _d, _d, _i, _d, _vp]),
+    "rg_kick": (_i, [_vp, _vp, _i64, _d, _vp]),
+    "rg_drift": (_i, [_vp, _vp, _i64, _d, _vp]),
+    "rg_kick_drift": (_i, [_vp, _vp, _vp, _i64, _d, _d, _vp]),
+    "rg_energy": (_i, [_vp, _vp, _vp, _i64, _d, _d, _vp, _vp]),
+    "rg_morton_codes": (_i, [_vp, _i64, _i, _vp, _vp]),
+    "rg_bh_create": (_i, [_i64, _d, _d, _d, _i, _i, ctypes.POINTER(_vp)]),
+    "rg_bh_compute": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "rg_bh_export_tree": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rg_bh_export_nodes": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "rg_bh_stats": (_i, [_vp, ctypes.POINTER(BHStats)]),
+    "rg_bh_set_timing": (_i, [_vp, _i]),
+    "rg_bh_timing": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_i64)]),
+    "rg_bh_destroy": (_i, [_vp]),
+    "rg_direct_vjp": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load() -> ctypes.CDLL:
+    """Load librgsim.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with "
+                    "`python real-astronomy-simulation_amd/build.py` (no CPU fallback exists)")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != RG_OK:
+        msg = load().rg_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what or 'rgsim'} failed (status {status}): {msg}")
+
+
+def precision_code(p) -> int:
+    if isinstance(p, int):
+        return p
+    try:
+        return PRECISIONS[p]
+    except KeyError:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {p!r}") from None
+
+
+def require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("rgsim computes on the GPU only; no GPU is visible")
+    load()
+
+
+def stream_ptr(t) -> int:
+    """Raw HIP stream of torch's current stream on t's device."""
+    import torch
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def ptr(t) -> int:
+    return t.data_ptr() if t is not None else None
+
+
+def set_param(name: str, value: int) -> None:
+    check(load().rg_set_param(name.encode(), int(value)), f"rg_set_param({name})")

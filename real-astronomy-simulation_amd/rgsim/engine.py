@@ -1,0 +1,140 @@
+"""Drop-in ``Engine`` for core/engine.py:32-124.
+
+Same constructor, methods and attributes as the reference's Engine
+(``Engine(softening, gravity_mode, theta)``; ``load_state``, ``to_state``,
+``step``, ``energies``; ``n``, ``time``, ``step_count``, ``seed``, ``meta``,
+``softening``, ``theta``, ``gravity_mode``, ``_bh``).  Fields live in torch
+float64 CUDA tensors and every kernel runs through the HIP C ABI:
+
+* Barnes-Hut is used iff ``gravity_mode == "bh" and n >= 256``
+  (core/engine.py:75); otherwise ``_bh is None`` and the direct sum runs.
+* ``step(dt)`` is the reference's KDK order (core/engine.py:108-116): the
+  opening kick and the drift are one fused pass, and in direct mode the
+  closing kick is fused into the force kernel.  Both fusions are
+  bit-identical to the separate reference kernels.
+* ``energies()`` is FP64 (KE, PE, KE + PE) like core/engine.py:119-124.
+
+Extra keyword arguments: ``precision`` ("fp64" default = the reference's
+arithmetic, "fp32", "fp32c"), ``G`` (default core.units.G), ``morton_bits``.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .integrators import leapfrog
+from .solvers import gravity
+from .solvers.barnes_hut import BarnesHut
+from .state import State
+from .units import G as G_GAL
+
+BH_MIN_N = 256
+
+
+def init_taichi(arch: str = "cuda") -> None:
+    """Compatibility shim for core/engine.py:21-29: there is no runtime to
+    initialise beyond selecting the GPU; ``arch`` other than a GPU is an
+    error because this solver has no CPU path."""
+    if arch not in ("cuda", "gpu", "hip", "rocm"):
+        raise RuntimeError(f"arch={arch!r}: rgsim runs on the GPU only")
+    _lib.require_gpu()
+
+
+class Engine:
+    def __init__(self, softening: float = 0.05, gravity_mode: str = "direct",
+                 theta: float = 0.6, *, precision: str = "fp64", G: float = G_GAL,
+                 morton_bits: int = 21, check_finite: bool = False):
+        self.softening = float(softening)
+        self.gravity_mode = gravity_mode
+        self.theta = float(theta)
+        self.precision = precision
+        self.G = float(G)
+        self.morton_bits = int(morton_bits)
+        self.check_finite = bool(check_finite)
+        _lib.precision_code(precision)
+        self._bh = None
+        self.n = 0
+        self.time = 0.0
+        self.step_count = 0
+        self.seed = 0
+        self.meta: dict = {}
+        self.pos = self.vel = self.acc = self.mass = None
+        self._ptype = self._ids = None
+        self._age = self._temperature = self._metallicity = None
+
+    # ------------------------------------------------------------- setup
+    def load_state(self, state: State) -> None:
+        """Upload a State and prime the accelerations (core/engine.py:53-81)."""
+        _lib.require_gpu()
+        self.n = state.n
+        self.time = state.time
+        self.step_count = state.step
+        self.seed = state.seed
+        self.meta = dict(state.meta)
+        self._ptype = state.ptype.copy()
+        self._ids = state.ids.copy()
+        self._age = state.age
+        self._temperature = state.temperature
+        self._metallicity = state.metallicity
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.pos = torch.from_numpy(np.ascontiguousarray(state.pos, np.float64)).to(dev)
+        self.vel = torch.from_numpy(np.ascontiguousarray(state.vel, np.float64)).to(dev)
+        self.mass = torch.from_numpy(np.ascontiguousarray(state.mass, np.float64)).to(dev)
+        self.acc = torch.empty_like(self.pos)
+        if self.gravity_mode == "bh" and self.n >= BH_MIN_N:
+            if self._bh is None or self._bh.n != self.n:
+                self._bh = BarnesHut(self.n, theta=self.theta, softening=self.softening,
+                                     G=self.G, precision=self.precision,
+                                     morton_bits=self.morton_bits)
+        else:
+            self._bh = None
+        self._compute_acc()
+
+    def to_state(self) -> State:
+        """Download the state (core/engine.py:83-98)."""
+        return State(pos=self.pos.cpu().numpy(), vel=self.vel.cpu().numpy(),
+                     mass=self.mass.cpu().numpy(), ptype=self._ptype, ids=self._ids,
+                     time=self.time, step=self.step_count, seed=self.seed, age=self._age,
+                     temperature=self._temperature, metallicity=self._metallicity,
+                     meta=dict(self.meta))
+
+    # ---------------------------------------------------------- stepping
+    def _compute_acc(self) -> None:
+        if self._bh is not None:
+            self._bh.compute(self.pos, self.mass, self.acc)
+            if self.check_finite:
+                self._check_bh()
+        else:
+            gravity.compute_acc_direct(self.pos, self.mass, self.acc, self.n, self.G,
+                                       self.softening ** 2, self.precision)
+
+    def _check_bh(self) -> None:
+        st = self._bh.stats()
+        if st["nonfinite"]:
+            raise FloatingPointError(f"{st['nonfinite']} non-finite coordinates in the input")
+
+    def step(self, dt: float) -> None:
+        """One KDK step (core/engine.py:108-116)."""
+        half = 0.5 * dt
+        leapfrog.kick_drift(self.pos, self.vel, self.acc, self.n, half, dt)
+        if self._bh is not None:
+            self._compute_acc()
+            leapfrog.kick(self.vel, self.acc, self.n, half)
+        else:
+            gravity.compute_acc_direct_kick(self.pos, self.mass, self.acc, self.vel, self.n,
+                                            self.G, self.softening ** 2, half, self.precision)
+        self.time += dt
+        self.step_count += 1
+
+    # ------------------------------------------------------- diagnostics
+    def energies(self) -> tuple[float, float, float]:
+        """(kinetic, potential, total), FP64 (core/engine.py:119-124)."""
+        ke, pe, _ = gravity.energy_terms(self.pos, self.vel, self.mass, self.n, self.G,
+                                         self.softening ** 2)
+        return ke, pe, ke + pe
+
+    def momentum(self):
+        """Total momentum sum m v (FP64)."""
+        return gravity.energy_terms(self.pos, self.vel, self.mass, self.n, self.G,
+                                    self.softening ** 2, with_pe=False)[2]

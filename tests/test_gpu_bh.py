@@ -1,0 +1,153 @@
+"""GPU parity of the Barnes-Hut pipeline against the oracle restatement of
+core/solvers/barnes_hut.py: Morton keys, stable order and topology
+bit-exact; aggregation to rounding (AABB exact); FP64 walk bit-exact given
+the same nodes; FP32 walk within the treecode's error."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def cases():
+    from rgsim.ic import make_disk_bulge, make_plummer
+    rng = np.random.default_rng(0)
+    out = {}
+    s = make_plummer(3000, total_mass=1.0, seed=42, G=1.0)
+    out["plummer3000"] = (s.pos, s.mass)
+    d = make_disk_bulge(n_disk=5000, n_bulge=1000, seed=1)
+    out["diskbulge6000"] = (d.pos, d.mass)
+    p = rng.normal(size=(1025, 3))
+    p[500:] = p[:525]  # duplicate positions -> duplicate keys
+    out["dups1025"] = (p, rng.uniform(0.5, 1.5, 1025))
+    out["n2"] = (np.array([[0.0, 0, 0], [1.0, 1, 1]]), np.array([1.0, 2.0]))
+    out["n3"] = (rng.normal(size=(3, 3)), np.ones(3))
+    q = rng.normal(size=(700, 3))
+    q[:, 2] = 0.0  # flat slab
+    out["slab700"] = (q, np.ones(700))
+    return out
+
+
+CASES = cases()
+
+
+def run_bh(pos, mass, theta=0.5, eps=0.05, G=1.0, precision="fp64", bits=21):
+    from rgsim.solvers.barnes_hut import BarnesHut
+    n = len(mass)
+    bh = BarnesHut(n, theta=theta, softening=eps, G=G, precision=precision, morton_bits=bits)
+    p, m = dev(pos), dev(mass)
+    acc = torch.empty_like(p)
+    bh.compute(p, m, acc)
+    return bh, acc.cpu().numpy()
+
+
+@pytest.mark.parametrize("bits", [21, 10])
+@pytest.mark.parametrize("name", list(CASES))
+def test_keys_order_topology_bitexact(orc, name, bits):
+    print(f"bh topology {name} bits={bits}", flush=True)
+    pos, mass = CASES[name]
+    bh, _ = run_bh(pos, mass, bits=bits)
+    keys, order, left, right, parent = (t.cpu().numpy() for t in bh.export_tree())
+    k_ref = orc.morton_c(pos, bits)
+    o_ref = np.argsort(k_ref, kind="stable").astype(np.int32)
+    assert np.array_equal(order, o_ref)
+    assert np.array_equal(keys.astype(np.uint64), k_ref[o_ref])
+    l_ref, r_ref, p_ref = orc.build_radix(k_ref[o_ref].astype(np.int64))
+    assert np.array_equal(left, l_ref)
+    assert np.array_equal(right, r_ref)
+    assert np.array_equal(parent, p_ref)
+
+
+def test_standalone_morton_codes(orc):
+    from rgsim.solvers.barnes_hut import morton_codes
+    for name, (pos, _) in CASES.items():
+        got = morton_codes(dev(pos)).cpu().numpy().astype(np.uint64)
+        assert np.array_equal(got, orc.morton_codes(pos)), name
+    corner = dev(np.array([[0.0, 0, 0], [1.0, 1, 1]]))
+    assert morton_codes(corner).cpu().numpy().tolist() == [0, 0x7FFFFFFFFFFFFFF8]
+
+
+@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000"])
+def test_aggregation(orc, name):
+    pos, mass = CASES[name]
+    bh, _ = run_bh(pos, mass)
+    com, nmass, nmin, nmax = (t.cpu().numpy() for t in bh.export_nodes())
+    _, order, _, _, parent = (t.cpu().numpy() for t in bh.export_tree())
+    c_ref, m_ref, mn_ref, mx_ref = orc.aggregate(order, pos, mass, parent)
+    assert np.array_equal(nmin, mn_ref) and np.array_equal(nmax, mx_ref)
+    np.testing.assert_allclose(nmass, m_ref, rtol=1e-13)
+    scale = np.abs(pos).max()
+    np.testing.assert_allclose(com, c_ref, rtol=1e-12, atol=1e-13 * scale)
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.3, 0.5, 0.8])
+@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3"])
+def test_fp64_walk_bitexact_given_nodes(orc, name, theta):
+    """The FP64 walk visits nodes in the reference's stack order with its
+    arithmetic: given the same node data the accelerations are bit-equal."""
+    print(f"bh walk fp64 {name} theta={theta}", flush=True)
+    pos, mass = CASES[name]
+    G, eps = 43009.1725, 0.05
+    bh, acc = run_bh(pos, mass, theta=theta, eps=eps, G=G)
+    com, nmass, nmin, nmax = (t.cpu().numpy() for t in bh.export_nodes())
+    _, order, left, right, _ = (t.cpu().numpy() for t in bh.export_tree())
+    ref, visits, ovf = orc.traverse(pos, order, com, nmass, nmin, nmax, left, right, G, theta,
+                                    eps, max_stack=4096)
+    assert ovf == 0
+    assert np.array_equal(acc, ref), np.abs(acc - ref).max()
+    st = bh.stats()
+    assert st["visits_total"] == visits.sum() and st["visits_max"] == visits.max()
+
+
+def test_fp64_bh_vs_full_oracle(orc):
+    pos, mass = CASES["plummer3000"]
+    _, acc = run_bh(pos, mass, theta=0.5, eps=0.01)
+    ref, _, _ = orc.bh_acc(pos, mass, 1.0, 0.5, 0.01)
+    rel = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert rel.max() < 1e-9  # only node-aggregation rounding differs
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_force_error_vs_direct(orc, precision):
+    pos, mass = CASES["diskbulge6000"]
+    G, eps = 43009.1725, 0.05
+    ref = orc.direct_acc(pos, mass, G, eps * eps)
+    mag = np.linalg.norm(ref, axis=1)
+    errs = []
+    for th in (0.0, 0.5, 0.6):
+        _, acc = run_bh(pos, mass, theta=th, eps=eps, G=G, precision=precision)
+        errs.append(np.mean(np.linalg.norm(acc - ref, axis=1) / mag))
+    assert errs[0] < (1e-5 if precision == "fp32" else 1e-12)
+    assert errs[1] < 0.01 and errs[2] < 0.05  # verify_bh_galaxy.py:54 bar is 5%
+
+
+def test_deterministic_repeat(orc):
+    pos, mass = CASES["diskbulge6000"]
+    from rgsim.solvers.barnes_hut import BarnesHut
+    for precision in ("fp64", "fp32"):
+        bh = BarnesHut(len(mass), theta=0.5, softening=0.05, precision=precision)
+        p, m = dev(pos), dev(mass)
+        a1, a2 = torch.empty_like(p), torch.empty_like(p)
+        bh.compute(p, m, a1)
+        n1 = [t.clone() for t in bh.export_nodes()]
+        bh.compute(p, m, a2)
+        n2 = bh.export_nodes()
+        assert torch.equal(a1, a2)
+        assert all(torch.equal(x, y) for x, y in zip(n1, n2))
+
+
+def test_nonfinite_is_flagged():
+    from rgsim.solvers.barnes_hut import BarnesHut
+    pos, mass = CASES["plummer3000"]
+    pos = pos.copy()
+    pos[17, 1] = np.nan
+    bh = BarnesHut(len(mass), theta=0.5, softening=0.05)
+    p, m = dev(pos), dev(mass)
+    acc = torch.empty_like(p)
+    bh.compute(p, m, acc)
+    st = bh.stats()
+    assert st["nonfinite"] == 1
