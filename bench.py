@@ -90,7 +90,8 @@ def bench_bh(steps, warmup):
         "visits_per_particle": visits / n, "visits_max": st_last["visits_max"],
         "visits_ic": st["visits_total"],
         "phase_ms": {"sort": tm["sort_ms"] / max(tm["count"], 1),
-                     "build_aggregate": tm["build_ms"] / max(tm["count"], 1),
+                     "build": tm["build_ms"] / max(tm["count"], 1),
+                     "aggregate": tm["aggregate_ms"] / max(tm["count"], 1),
                      "walk": walk_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

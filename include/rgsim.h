@@ -146,12 +146,12 @@ int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out);
 
 /* Per-phase timing with HIP events recorded on the compute's stream (no
  * reference counterpart; the reference measures wall time only).  While
- * enabled, each rg_bh_compute records 4 events (up to 4096 computes).
+ * enabled, each rg_bh_compute records 5 events (up to 4096 computes).
  * rg_bh_timing waits for the recorded computes and returns the summed
- * milliseconds of {bbox+Morton+sort, build+aggregation, walk} and how many
+ * milliseconds of {bbox+Morton+sort, build, aggregation, walk} and how many
  * computes they cover, then resets the count. */
 int rg_bh_set_timing(rg_bh *h, int enable);
-int rg_bh_timing(rg_bh *h, double *ms3, int64_t *count);
+int rg_bh_timing(rg_bh *h, double *ms4, int64_t *count);
 
 int rg_bh_destroy(rg_bh *h);
 

@@ -11,14 +11,18 @@
 //                  permutation equals np.argsort(kind="stable") (:247)
 //   4. k_build     Karras build with the reference's exact delta (incl. the
 //                  index tie-break) using hardware clz (:50-121)
-//   5. k_aggregate leaf init + deterministic bottom-up aggregation: the
-//                  second child to finish combines both children (acquire /
-//                  release fences around an arrival counter), so every node
-//                  is computed exactly once and bitwise reproducibly (:124-177)
-//   6. k_walk      stackless right-first DFS with escape pointers: the same
-//                  node sequence the reference's stack walk pops (:180-217),
-//                  over node records laid out for one 32-byte (FP32) or
-//                  48-byte (FP64) fetch per visit.
+//   5. k_agg_*     leaf init + deterministic bottom-up aggregation: the
+//                  second child to finish combines both children (acq_rel
+//                  arrival counters; LDS and workgroup scope inside a chunk
+//                  of leaves, one workgroup for the nodes that straddle
+//                  chunks), so every node is computed exactly once and
+//                  bitwise reproducibly (:124-177)
+//   6. k_walk*     stackless walk in right-first DFS order: the node
+//                  sequence the reference's stack walk pops (:180-217).
+//                  FP64 follows right-child/escape links over node-id
+//                  records; FP32 scans records stored in that preorder,
+//                  moving to the next slot to open a node and skipping the
+//                  subtree to accept it.
 //
 // Node ids follow the reference: leaves 0..n-1 in sorted order, internal
 // node i has id i+n, root id n.
@@ -40,14 +44,6 @@ constexpr int kBoxBS = 256;
 constexpr int kWalkBSDefault = 256;
 
 int g_walk_bs = kWalkBSDefault;
-
-struct alignas(16) WalkRec32 {
-    float x, y, z, gm;   // centre of mass, G * mass
-    float crit;          // accept when crit < r2 (leaves: -inf)
-    int right;           // right child id (leaves: -1)
-    int escape;          // next node in right-first DFS order after this subtree (-1: done)
-    int pad;
-};
 
 struct alignas(16) WalkRec64 {
     double x, y, z, m;   // centre of mass, mass
@@ -337,10 +333,13 @@ __device__ __forceinline__ int delta_dev(int64_t i, int64_t j, int64_t n, uint64
     return __clzll(static_cast<long long>(x)) - 1;
 }
 
+// Upward links for the preorder pass: up[v] = (parent id << 1) | (v is the
+// right child); up[root] = -1.
 __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restrict__ left,
                         int *__restrict__ right, int *__restrict__ parent,
-                        int *__restrict__ lo_range, int *__restrict__ left_at_split,
-                        int *__restrict__ arrive)
+                        int *__restrict__ up, int2 *__restrict__ range,
+                        int *__restrict__ left_at_split, int *__restrict__ arrive,
+                        int *__restrict__ wl_count)
 {
     const int64_t nn = n;
     for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < n - 1;
@@ -368,14 +367,46 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
         const int64_t hi = i < j ? j : i;
         const int lc = static_cast<int>(lo == gamma ? gamma : gamma + nn);
         const int rc = static_cast<int>(hi == gamma + 1 ? gamma + 1 : gamma + 1 + nn);
+        const int self = static_cast<int>(i + nn);
         left[i] = lc;
         right[i] = rc;
-        parent[lc] = static_cast<int>(i + nn);
-        parent[rc] = static_cast<int>(i + nn);
-        lo_range[i] = static_cast<int>(lo);
+        parent[lc] = self;
+        parent[rc] = self;
+        up[lc] = self << 1;
+        up[rc] = (self << 1) | 1;
+        range[i] = make_int2(static_cast<int>(lo), static_cast<int>(hi));
         left_at_split[gamma] = lc;
         arrive[i] = 0;
-        if (i == 0) parent[nn] = -1;
+        if (i == 0) {
+            parent[nn] = -1;
+            up[nn] = -1;
+            *wl_count = 0;
+        }
+    }
+}
+
+// Right-first DFS preorder index of every node (the order the reference's
+// stack walk pops nodes in): pre(v) = 2 (n-1-hi(v)) + (right turns on the
+// root-to-v path), hi = last leaf of v's range.  The subtree of v then
+// occupies [pre(v), pre(v) + 2 (hi-lo+1) - 1).  stats[3] flags a chain
+// longer than any valid tree (corrupt links).
+constexpr int kMaxDepth = 128;  // > 63 + 31 + 1, the deepest tree the tie-break allows
+
+__global__ void k_preorder(const int *__restrict__ up, const int2 *__restrict__ range, int n,
+                           int *__restrict__ pre, unsigned long long *__restrict__ stats)
+{
+    const int m = 2 * n - 1;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m; v += gridDim.x * blockDim.x) {
+        const int hi = v < n ? v : range[v - n].y;
+        int r = 0, u = v, steps = 0;
+        while (u != n && u >= 0 && u < m && steps < kMaxDepth) {
+            const int x = up[u];
+            r += x & 1;
+            u = x >> 1;
+            ++steps;
+        }
+        if (u != n) atomicAdd(&stats[3], 1ull);
+        pre[v] = 2 * (n - 1 - hi) + r;
     }
 }
 
@@ -384,106 +415,175 @@ struct NodeArrays {
     double *com, *nmass, *nmin, *nmax, *nmom;
 };
 
-template <int MODE>  // 0: FP64 walk records, 1: FP32 walk records
-__global__ void k_aggregate(const int *__restrict__ order, const double *__restrict__ pos,
-                            const double *__restrict__ mass, int n,
-                            const int *__restrict__ left, const int *__restrict__ right,
-                            const int *__restrict__ parent, const int *__restrict__ lo_range,
-                            const int *__restrict__ left_at_split, int *__restrict__ arrive,
-                            NodeArrays na, void *__restrict__ recs, double G, double inv_theta2,
-                            int theta_zero)
-{
-    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n;
-         li += gridDim.x * blockDim.x) {
-        const int p = order[li];
-        const double x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
-        const double m = mass[p];
-        // barnes_hut.py:129-134 (leaf init).
-        na.com[3 * li] = x;
-        na.com[3 * li + 1] = y;
-        na.com[3 * li + 2] = z;
-        na.nmin[3 * li] = x;
-        na.nmin[3 * li + 1] = y;
-        na.nmin[3 * li + 2] = z;
-        na.nmax[3 * li] = x;
-        na.nmax[3 * li + 1] = y;
-        na.nmax[3 * li + 2] = z;
-        na.nmass[li] = m;
-        na.nmom[3 * li] = m * x;
-        na.nmom[3 * li + 1] = m * y;
-        na.nmom[3 * li + 2] = m * z;
-        const int esc_leaf = li == 0 ? -1 : left_at_split[li - 1];
-        if (MODE == 0) {
-            WalkRec64 r;
-            r.x = x; r.y = y; r.z = z; r.m = m;
-            r.size = 0.0;
-            r.right = -1;
-            r.escape = esc_leaf;
-            static_cast<WalkRec64 *>(recs)[li] = r;
-        } else {
-            WalkRec32 r;
-            r.x = static_cast<float>(x);
-            r.y = static_cast<float>(y);
-            r.z = static_cast<float>(z);
-            r.gm = static_cast<float>(G * m);
-            r.crit = -INFINITY;
-            r.right = -1;
-            r.escape = esc_leaf;
-            r.pad = 0;
-            static_cast<WalkRec32 *>(recs)[li] = r;
-        }
-        __threadfence();  // release the leaf before signalling its parent
+// Walk records.  FP64 (reference parity): indexed by node id, with the
+// right child and the escape (next node in right-first DFS order after the
+// subtree).  FP32: stored in right-first DFS preorder, so opening a node
+// moves to the next slot and accepting it skips its subtree: recA = (com,
+// G*mass), recB = (accept threshold, skip slot); leaves have threshold -inf.
+struct WalkOut {
+    WalkRec64 *rec64;
+    float4 *recA;
+    int2 *recB;
+    const int *pre;
+    double G, inv_theta2;
+    int theta_zero;
+};
 
-        int node = parent[li];
+template <int MODE>
+__device__ __forceinline__ void write_node(const NodeArrays &na, const WalkOut &wo, int node,
+                                           int n, double m, const double mom[3],
+                                           const double mn[3], const double mx[3],
+                                           const double c[3], int lo, int hi, int right_child,
+                                           int escape)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        na.nmom[3 * node + a] = mom[a];
+        na.nmin[3 * node + a] = mn[a];
+        na.nmax[3 * node + a] = mx[a];
+        na.com[3 * node + a] = c[a];
+    }
+    na.nmass[node] = m;
+    double size = mx[0] - mn[0];
+    size = mx[1] - mn[1] > size ? mx[1] - mn[1] : size;
+    size = mx[2] - mn[2] > size ? mx[2] - mn[2] : size;
+    const bool leaf = node < n;
+    if (MODE == 0) {
+        WalkRec64 r;
+        r.x = c[0]; r.y = c[1]; r.z = c[2]; r.m = m;
+        r.size = leaf ? 0.0 : size;
+        r.right = leaf ? -1 : right_child;
+        r.escape = escape;
+        wo.rec64[node] = r;
+    } else {
+        const int p = wo.pre[node];
+        const float crit = leaf ? -INFINITY
+                                : (wo.theta_zero ? INFINITY
+                                                 : static_cast<float>(size * size * wo.inv_theta2));
+        wo.recA[p] = make_float4(static_cast<float>(c[0]), static_cast<float>(c[1]),
+                                 static_cast<float>(c[2]), static_cast<float>(wo.G * m));
+        wo.recB[p] = make_int2(__float_as_int(crit), p + 2 * (hi - lo + 1) - 1);
+    }
+}
+
+constexpr int kAggChunk = 256;  // leaves per workgroup in the aggregation
+
+// Pass 1: one thread per sorted leaf, one workgroup per chunk of kAggChunk
+// leaves.  A node whose leaf range lies inside the chunk has both children
+// finished by this workgroup: it is combined from LDS, and its arrival
+// counter is an LDS atomic at workgroup scope.  A thread that reaches a
+// node straddling chunks stops and queues that node for pass 2.
+template <int MODE>
+__global__ __launch_bounds__(kAggChunk) void k_agg_chunk(
+    const int *__restrict__ order, const double *__restrict__ pos,
+    const double *__restrict__ mass, int n, const int *__restrict__ left,
+    const int *__restrict__ right, const int *__restrict__ parent,
+    const int2 *__restrict__ range, const int *__restrict__ left_at_split,
+    int *__restrict__ worklist, int *__restrict__ wl_count, NodeArrays na, WalkOut wo)
+{
+    constexpr int C = kAggChunk;
+    __shared__ double s_m[2 * C];
+    __shared__ double s_mom[3][2 * C];
+    __shared__ double s_mn[3][2 * C];
+    __shared__ double s_mx[3][2 * C];
+    __shared__ int s_arrive[C];
+    const int c0 = blockIdx.x * C;
+    s_arrive[threadIdx.x] = 0;
+    __syncthreads();
+    const int li = c0 + threadIdx.x;
+    if (li >= n) return;  // no barrier follows
+
+    const int p = order[li];
+    const double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+    const double m = mass[p];
+    // barnes_hut.py:129-134 (leaf init).
+    double mom[3];
+    const int sl = li - c0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        mom[a] = m * x[a];
+        s_mom[a][sl] = mom[a];
+        s_mn[a][sl] = x[a];
+        s_mx[a][sl] = x[a];
+    }
+    s_m[sl] = m;
+    write_node<MODE>(na, wo, li, n, m, mom, x, x, x, li, li, -1,
+                     li == 0 ? -1 : left_at_split[li - 1]);
+
+    int node = parent[li];
+    while (node >= 0) {
+        const int ii = node - n;
+        const int2 rg = range[ii];
+        if (rg.x < c0 || rg.y >= c0 + C) {
+            worklist[atomicAdd(wl_count, 1)] = node;
+            break;
+        }
+        // acq_rel: publishes this thread's LDS/global node, acquires the sibling's.
+        if (__hip_atomic_fetch_add(&s_arrive[ii - c0], 1, __ATOMIC_ACQ_REL,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+            break;  // the sibling subtree is not finished; its thread continues
+        const int L = left[ii], R = right[ii];
+        const int sL = L < n ? L - c0 : C + (L - n - c0);
+        const int sR = R < n ? R - c0 : C + (R - n - c0);
+        const double mm = s_m[sL] + s_m[sR];
+        double mn[3], mx[3], c[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            mom[a] = s_mom[a][sL] + s_mom[a][sR];
+            mn[a] = fmin(s_mn[a][sL], s_mn[a][sR]);
+            mx[a] = fmax(s_mx[a][sL], s_mx[a][sR]);
+            // barnes_hut.py:175-177: com = mom / mass where mass > 0.
+            c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
+        }
+        const int sn = C + ii - c0;
+        s_m[sn] = mm;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            s_mom[a][sn] = mom[a];
+            s_mn[a][sn] = mn[a];
+            s_mx[a][sn] = mx[a];
+        }
+        write_node<MODE>(na, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
+                         rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
+        node = parent[node];
+    }
+}
+
+// Pass 2: one workgroup finishes the nodes that straddle chunks, starting
+// from the queued arrivals.  Children finished in pass 1 are visible across
+// the kernel boundary; children finished here are published by the
+// workgroup-scope acq_rel arrival counter.
+constexpr int kUpperBS = 1024;
+
+template <int MODE>
+__global__ __launch_bounds__(kUpperBS) void k_agg_upper(
+    int n, const int *__restrict__ left, const int *__restrict__ right,
+    const int *__restrict__ parent, const int2 *__restrict__ range,
+    const int *__restrict__ left_at_split, int *__restrict__ arrive,
+    const int *__restrict__ worklist, const int *__restrict__ wl_count, NodeArrays na,
+    WalkOut wo)
+{
+    const int cnt = *wl_count;
+    for (int e = threadIdx.x; e < cnt; e += kUpperBS) {
+        int node = worklist[e];
         while (node >= 0) {
             const int ii = node - n;
-            const int old = atomicAdd(&arrive[ii], 1);
-            if (old == 0) break;  // the sibling subtree is not finished; it will continue
-            __threadfence();      // acquire: the sibling's stores are visible
+            if (__hip_atomic_fetch_add(&arrive[ii], 1, __ATOMIC_ACQ_REL,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
             const int L = left[ii], R = right[ii];
-            const double mL = na.nmass[L], mR = na.nmass[R];
-            const double mm = mL + mR;
+            const int2 rg = range[ii];
+            const double mm = na.nmass[L] + na.nmass[R];
             double mom[3], mn[3], mx[3], c[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 mom[a] = na.nmom[3 * L + a] + na.nmom[3 * R + a];
                 mn[a] = fmin(na.nmin[3 * L + a], na.nmin[3 * R + a]);
                 mx[a] = fmax(na.nmax[3 * L + a], na.nmax[3 * R + a]);
-                // barnes_hut.py:175-177: com = mom / mass where mass > 0.
                 c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
-                na.nmom[3 * node + a] = mom[a];
-                na.nmin[3 * node + a] = mn[a];
-                na.nmax[3 * node + a] = mx[a];
-                na.com[3 * node + a] = c[a];
             }
-            na.nmass[node] = mm;
-            double size = mx[0] - mn[0];
-            const double sy = mx[1] - mn[1];
-            const double sz = mx[2] - mn[2];
-            size = sy > size ? sy : size;
-            size = sz > size ? sz : size;
-            const int lo = lo_range[ii];
-            const int esc = lo == 0 ? -1 : left_at_split[lo - 1];
-            if (MODE == 0) {
-                WalkRec64 r;
-                r.x = c[0]; r.y = c[1]; r.z = c[2]; r.m = mm;
-                r.size = size;
-                r.right = R;
-                r.escape = esc;
-                static_cast<WalkRec64 *>(recs)[node] = r;
-            } else {
-                WalkRec32 r;
-                r.x = static_cast<float>(c[0]);
-                r.y = static_cast<float>(c[1]);
-                r.z = static_cast<float>(c[2]);
-                r.gm = static_cast<float>(G * mm);
-                r.crit = theta_zero ? INFINITY : static_cast<float>(size * size * inv_theta2);
-                r.right = R;
-                r.escape = esc;
-                r.pad = 0;
-                static_cast<WalkRec32 *>(recs)[node] = r;
-            }
-            __threadfence();  // release this node before signalling its parent
+            write_node<MODE>(na, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
+                             rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
             node = parent[node];
         }
     }
@@ -515,7 +615,8 @@ __device__ __forceinline__ int wave_max_i32(int v)
     return v;
 }
 
-// FP64: the reference's criterion and arithmetic (barnes_hut.py:196-216).
+// FP64: the reference's criterion and arithmetic (barnes_hut.py:196-216),
+// nodes in the reference's pop order via right child / escape links.
 __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restrict__ order,
                          double *__restrict__ acc, int n, double G, double theta2, double eps2,
                          unsigned long long *__restrict__ stats)
@@ -564,7 +665,12 @@ __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restric
     }
 }
 
-__global__ void k_walk32(const WalkRec32 *__restrict__ rec, const int *__restrict__ order,
+// FP32: a forward scan of the preorder records with subtree skips.  Slot
+// k+1 is the right child of an opened internal slot k; skip is the first
+// slot after the subtree.  Every step moves forward, so a walk ends after at
+// most 2n-1 steps.
+__global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict__ recB,
+                         const int *__restrict__ pre, const int *__restrict__ order,
                          double *__restrict__ acc, int n, float eps2,
                          unsigned long long *__restrict__ stats)
 {
@@ -573,32 +679,30 @@ __global__ void k_walk32(const WalkRec32 *__restrict__ rec, const int *__restric
     const bool active = t < n;
     int visits = 0;
     if (active) {
-        const float4 self = *reinterpret_cast<const float4 *>(&rec[t]);
+        const float4 self = recA[pre[t]];
         const float px = self.x, py = self.y, pz = self.z;
         float ax = 0.f, ay = 0.f, az = 0.f;
-        int node = n;
-        const int max_it = 2 * n;
         const int m = 2 * n - 1;
-        while (node >= 0 && node < m && visits < max_it) {
+        int k = 0;  // the root's slot
+        while (k < m) {
             ++visits;
-            const float4 c = *reinterpret_cast<const float4 *>(&rec[node]);
-            const int4 a = *(reinterpret_cast<const int4 *>(&rec[node]) + 1);
+            const float4 c = recA[k];
+            const int2 sb = recB[k];
             const float dx = c.x - px;
             const float dy = c.y - py;
             const float dz = c.z - pz;
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-            if (__int_as_float(a.x) < r2) {
+            if (__int_as_float(sb.x) < r2) {
                 const float inv = __builtin_amdgcn_rsqf(r2);
                 const float s = c.w * (inv * inv) * inv;
                 ax = fmaf(s, dx, ax);
                 ay = fmaf(s, dy, ay);
                 az = fmaf(s, dz, az);
-                node = a.z;
+                k = max(sb.y, k + 1);
             } else {
-                node = a.y;
+                k = k + 1;
             }
         }
-        if (node >= 0) atomicAdd(&stats[3], 1ull);
         const int p = order[t];
         acc[3 * p] = static_cast<double>(ax);
         acc[3 * p + 1] = static_cast<double>(ay);
@@ -631,14 +735,18 @@ struct rg_bh {
     int *idx[2] = {nullptr, nullptr};
     unsigned *hist = nullptr;
     int *left = nullptr, *right = nullptr, *parent = nullptr;
-    int *lo_range = nullptr, *left_at_split = nullptr, *arrive = nullptr;
+    int2 *range = nullptr;
+    int *left_at_split = nullptr, *arrive = nullptr;
+    int *up = nullptr, *pre = nullptr, *worklist = nullptr, *wl_count = nullptr;
     NodeArrays na{};
-    void *recs = nullptr;
+    WalkRec64 *rec64 = nullptr;  // FP64 walk records (node-id order)
+    float4 *recA = nullptr;      // FP32 walk records (preorder)
+    int2 *recB = nullptr;
     double *box = nullptr;    // bbox partials followed by the 4-double box
     unsigned long long *stats = nullptr;
     std::vector<void *> allocs;
 
-    // Optional per-phase timing: 4 events per compute (start, sorted,
+    // Optional per-phase timing: 5 events per compute (start, sorted, built,
     // aggregated, walked), up to kTimingCap computes between reads.
     static constexpr int kTimingCap = 4096;
     bool timing = false;
@@ -666,7 +774,7 @@ void bh_free(rg_bh *h)
 int bh_mark(rg_bh *h, int phase, hipStream_t st)
 {
     if (!h->timing || h->timed >= rg_bh::kTimingCap) return RG_OK;
-    RG_HIP(hipEventRecord(h->ev[4 * h->timed + phase], st));
+    RG_HIP(hipEventRecord(h->ev[5 * h->timed + phase], st));
     return RG_OK;
 }
 
@@ -728,7 +836,7 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->hist, 4 * 256 * static_cast<size_t>(h->nblk));
     ALLOC(h->left, 4 * (N - 1));
     ALLOC(h->right, 4 * (N - 1));
-    ALLOC(h->lo_range, 4 * (N - 1));
+    ALLOC(h->range, 8 * (N - 1));
     ALLOC(h->left_at_split, 4 * (N - 1));
     ALLOC(h->arrive, 4 * (N - 1));
     ALLOC(h->parent, 4 * M);
@@ -737,7 +845,16 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->na.nmin, 24 * M);
     ALLOC(h->na.nmax, 24 * M);
     ALLOC(h->na.nmom, 24 * M);
-    ALLOC(h->recs, (precision == RG_FP64 ? sizeof(WalkRec64) : sizeof(WalkRec32)) * M);
+    ALLOC(h->up, 4 * M);
+    ALLOC(h->worklist, 4 * 2 * N);
+    ALLOC(h->wl_count, 16);
+    if (precision == RG_FP64) {
+        ALLOC(h->rec64, sizeof(WalkRec64) * M);
+    } else {
+        ALLOC(h->pre, 4 * M);
+        ALLOC(h->recA, 16 * M);
+        ALLOC(h->recB, 8 * M);
+    }
     ALLOC(h->box, 8 * bbox_part_doubles(static_cast<int>(n)));
     ALLOC(h->stats, 8 * 4);
 #undef ALLOC
@@ -788,38 +905,60 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
     const int *order = h->idx[cur];
 
     const int g1 = min((n + 255) / 256, 16384);
-    k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->lo_range,
-                                h->left_at_split, h->arrive);
+    k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->up, h->range,
+                                h->left_at_split, h->arrive, h->wl_count);
     RG_CHECK_LAUNCH();
+    const bool fp64 = h->precision == RG_FP64;
+    if (!fp64) {
+        const int gp = min((2 * n - 1 + 255) / 256, 16384);
+        k_preorder<<<gp, 256, 0, st>>>(h->up, h->range, n, h->pre, h->stats);
+        RG_CHECK_LAUNCH();
+    }
 
     const double theta2 = h->theta * h->theta;
-    const int theta_zero = !(theta2 > 0.0);
-    const double inv_theta2 = theta_zero ? 0.0 : 1.0 / theta2;
-    if (h->precision == RG_FP64) {
-        k_aggregate<0><<<g1, 256, 0, st>>>(order, pos, mass, n, h->left, h->right, h->parent,
-                                           h->lo_range, h->left_at_split, h->arrive, h->na,
-                                           h->recs, h->G, inv_theta2, theta_zero);
+    WalkOut wo;
+    wo.rec64 = h->rec64;
+    wo.recA = h->recA;
+    wo.recB = h->recB;
+    wo.pre = h->pre;
+    wo.G = h->G;
+    wo.theta_zero = !(theta2 > 0.0);
+    wo.inv_theta2 = wo.theta_zero ? 0.0 : 1.0 / theta2;
+    rc = bh_mark(h, 2, st);
+    if (rc != RG_OK) return rc;
+    const int ga = (n + kAggChunk - 1) / kAggChunk;
+    if (fp64) {
+        k_agg_chunk<0><<<ga, kAggChunk, 0, st>>>(order, pos, mass, n, h->left, h->right,
+                                                h->parent, h->range, h->left_at_split,
+                                                h->worklist, h->wl_count, h->na, wo);
+        RG_CHECK_LAUNCH();
+        k_agg_upper<0><<<1, kUpperBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,
+                                               h->left_at_split, h->arrive, h->worklist,
+                                               h->wl_count, h->na, wo);
     } else {
-        k_aggregate<1><<<g1, 256, 0, st>>>(order, pos, mass, n, h->left, h->right, h->parent,
-                                           h->lo_range, h->left_at_split, h->arrive, h->na,
-                                           h->recs, h->G, inv_theta2, theta_zero);
+        k_agg_chunk<1><<<ga, kAggChunk, 0, st>>>(order, pos, mass, n, h->left, h->right,
+                                                h->parent, h->range, h->left_at_split,
+                                                h->worklist, h->wl_count, h->na, wo);
+        RG_CHECK_LAUNCH();
+        k_agg_upper<1><<<1, kUpperBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,
+                                               h->left_at_split, h->arrive, h->worklist,
+                                               h->wl_count, h->na, wo);
     }
     RG_CHECK_LAUNCH();
-    rc = bh_mark(h, 2, st);
+    rc = bh_mark(h, 3, st);
     if (rc != RG_OK) return rc;
 
     const int wb = g_walk_bs;
     const int wg = (n + wb - 1) / wb;
     const double eps2 = h->eps * h->eps;
-    if (h->precision == RG_FP64) {
-        k_walk64<<<wg, wb, 0, st>>>(static_cast<const WalkRec64 *>(h->recs), order, acc, n, h->G,
-                                    theta2, eps2, h->stats);
+    if (fp64) {
+        k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, acc, n, h->G, theta2, eps2, h->stats);
     } else {
-        k_walk32<<<wg, wb, 0, st>>>(static_cast<const WalkRec32 *>(h->recs), order, acc, n,
+        k_walk32<<<wg, wb, 0, st>>>(h->recA, h->recB, h->pre, order, acc, n,
                                     static_cast<float>(eps2), h->stats);
     }
     RG_CHECK_LAUNCH();
-    rc = bh_mark(h, 3, st);
+    rc = bh_mark(h, 4, st);
     if (rc != RG_OK) return rc;
     if (h->timing && h->timed < rg_bh::kTimingCap) ++h->timed;
     h->last_stream = st;
@@ -831,7 +970,7 @@ extern "C" int rg_bh_set_timing(rg_bh *h, int enable)
 {
     RG_REQUIRE(h, RG_ERR_ARG, "null handle");
     if (enable && h->ev.empty()) {
-        h->ev.resize(4 * rg_bh::kTimingCap, nullptr);
+        h->ev.resize(5 * rg_bh::kTimingCap, nullptr);
         for (auto &e : h->ev) RG_HIP(hipEventCreate(&e));
     }
     h->timing = enable != 0;
@@ -839,18 +978,18 @@ extern "C" int rg_bh_set_timing(rg_bh *h, int enable)
     return RG_OK;
 }
 
-extern "C" int rg_bh_timing(rg_bh *h, double *ms3, int64_t *count)
+extern "C" int rg_bh_timing(rg_bh *h, double *ms4, int64_t *count)
 {
-    RG_REQUIRE(h && ms3 && count, RG_ERR_ARG, "null argument");
-    ms3[0] = ms3[1] = ms3[2] = 0.0;
+    RG_REQUIRE(h && ms4 && count, RG_ERR_ARG, "null argument");
+    ms4[0] = ms4[1] = ms4[2] = ms4[3] = 0.0;
     *count = h->timed;
     for (int k = 0; k < h->timed; ++k) {
-        hipEvent_t *e = &h->ev[4 * k];
-        RG_HIP(hipEventSynchronize(e[3]));
-        for (int ph = 0; ph < 3; ++ph) {
+        hipEvent_t *e = &h->ev[5 * k];
+        RG_HIP(hipEventSynchronize(e[4]));
+        for (int ph = 0; ph < 4; ++ph) {
             float ms = 0.f;
             RG_HIP(hipEventElapsedTime(&ms, e[ph], e[ph + 1]));
-            ms3[ph] += ms;
+            ms4[ph] += ms;
         }
     }
     h->timed = 0;

@@ -109,10 +109,11 @@ class BarnesHut:
 
     def timing(self) -> dict:
         """Summed per-phase milliseconds since the last read (waits for them)."""
-        ms = (ctypes.c_double * 3)()
+        ms = (ctypes.c_double * 4)()
         cnt = ctypes.c_int64()
         _lib.check(_lib.load().rg_bh_timing(self._h, ms, ctypes.byref(cnt)), "rg_bh_timing")
-        return {"count": cnt.value, "sort_ms": ms[0], "build_ms": ms[1], "walk_ms": ms[2]}
+        return {"count": cnt.value, "sort_ms": ms[0], "build_ms": ms[1],
+                "aggregate_ms": ms[2], "walk_ms": ms[3]}
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:

@@ -121,8 +121,13 @@ def test_force_error_vs_direct(orc, precision):
     for th in (0.0, 0.5, 0.6):
         _, acc = run_bh(pos, mass, theta=th, eps=eps, G=G, precision=precision)
         errs.append(np.mean(np.linalg.norm(acc - ref, axis=1) / mag))
+    oracle_05 = orc.bh_acc(pos, mass, G, 0.5, eps)[0]
+    err_oracle = np.mean(np.linalg.norm(oracle_05 - ref, axis=1) / mag)
+    print(f"{precision}: mean rel err theta 0/0.5/0.6 = {errs}, oracle(0.5) {err_oracle}")
     assert errs[0] < (1e-5 if precision == "fp32" else 1e-12)
-    assert errs[1] < 0.01 and errs[2] < 0.05  # verify_bh_galaxy.py:54 bar is 5%
+    # the multipole error is the reference algorithm's own; FP32 adds ~1e-6
+    assert abs(errs[1] - err_oracle) < 5e-2 * err_oracle + 1e-5
+    assert errs[1] < 0.03 and errs[2] < 0.05  # verify_bh_galaxy.py:54 bar is 5%
 
 
 def test_deterministic_repeat(orc):
@@ -151,3 +156,19 @@ def test_nonfinite_is_flagged():
     bh.compute(p, m, acc)
     st = bh.stats()
     assert st["nonfinite"] == 1
+
+
+@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3"])
+def test_fp32_walk_visits_match_reference_walk(orc, name):
+    """The FP32 preorder scan examines the nodes the reference's stack walk
+    pops: exactly 2n-1 per particle at theta 0, and (FP32 rounding of the
+    criterion aside) the same total at theta 0.5."""
+    pos, mass = CASES[name]
+    n = len(mass)
+    bh, _ = run_bh(pos, mass, theta=0.0, eps=0.05, precision="fp32")
+    st = bh.stats()
+    assert st["visits_total"] == n * (2 * n - 1) and st["visits_max"] == 2 * n - 1
+    bh, _ = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
+    _, visits, ovf = orc.bh_acc(pos, mass, 1.0, 0.5, 0.05, max_stack=4096)
+    st = bh.stats()
+    assert abs(st["visits_total"] - visits.sum()) <= 1e-3 * visits.sum() + 2
