@@ -167,6 +167,58 @@ int rg_direct_vjp(const double *pos, const double *mass, const double *lam,
                   double *g_pos, double *g_mass, int64_t n, double G, double eps2,
                   int precision, void *stream);
 
+/* Elementwise part of the reverse of one KDK step (no reference
+ * counterpart).  With g_pos/g_mass the VJP of the acceleration at the
+ * step's end positions:  xbar += g_pos;  vbar += xbar * dt;
+ * mbar += g_mass;  lam = vbar * lam_coef  (the VJP weight for the previous
+ * step: lam_coef = dt inside a trajectory, dt/2 for the initial
+ * acceleration).  All arrays (n,3) except mbar/g_mass (n). */
+int rg_kdk_adjoint_update(double *xbar, double *vbar, double *mbar, const double *g_pos,
+                          const double *g_mass, double *lam, int64_t n, double dt,
+                          double lam_coef, void *stream);
+
+/* ------------------------------------------------------------ batching */
+
+/* An ensemble of independent systems advanced together (no reference
+ * counterpart; the reference runs one system per Engine, core/engine.py:32).
+ * System k owns particles [offsets[k], offsets[k+1]) of the concatenated
+ * (Ntot,3)/(Ntot) FP64 arrays; G, eps2 (> 0) and dt are per system
+ * (HOST arrays of nsys, copied).  precision RG_FP64 or RG_FP32C; each
+ * system's accelerations are bitwise those of rg_direct_acc on that system
+ * alone.  Not thread-safe per handle. */
+typedef struct rg_batch rg_batch;
+int rg_batch_create(int64_t nsys, const int64_t *offsets, const double *G,
+                    const double *eps2, const double *dt, int precision, rg_batch **out);
+
+/* Launch shape (the autotuner's knobs): tile = i-particles per work item =
+ * threads per workgroup (64, 128, 256); grid = persistent workgroups
+ * (0 = 4 per CU).  Drops a captured graph. */
+int rg_batch_set_shape(rg_batch *h, int tile, int grid);
+
+/* Accelerations of every system (overwrites acc). */
+int rg_batch_acc(rg_batch *h, const double *pos, const double *mass, double *acc,
+                 void *stream);
+
+/* nsteps KDK steps of every system with its own dt: per step one
+ * kick+drift launch and one persistent force(+closing kick) launch.  acc
+ * must hold the accelerations of the current positions on entry (as
+ * Engine.step assumes, core/engine.py:108-116). */
+int rg_batch_step(rg_batch *h, double *pos, double *vel, double *acc, const double *mass,
+                  int64_t nsteps, void *stream);
+
+/* Capture nsteps steps on these buffers into a hipGraph (replacing any
+ * previous one); rg_batch_graph_launch replays it on `stream`. */
+int rg_batch_graph_capture(rg_batch *h, double *pos, double *vel, double *acc,
+                           const double *mass, int64_t nsteps);
+int rg_batch_graph_launch(rg_batch *h, void *stream);
+
+/* Per-system FP64 {KE, PE, Px, Py, Pz} into out (device, nsys x 5), fixed
+ * reduction order. */
+int rg_batch_energy(rg_batch *h, const double *pos, const double *vel, const double *mass,
+                    double *out, void *stream);
+
+int rg_batch_destroy(rg_batch *h);
+
 #ifdef __cplusplus
 }
 #endif

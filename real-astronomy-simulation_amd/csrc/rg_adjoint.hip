@@ -196,3 +196,48 @@ extern "C" int rg_direct_vjp(const double *pos, const double *mass, const double
     RG_CHECK_LAUNCH();
     return RG_OK;
 }
+
+// ---------------------------------------------------------------------
+// Reverse of one leapfrog KDK step (core/engine.py:108-116), elementwise
+// part.  Forward:  v_h = v + a h;  x' = x + v_h dt;  a' = A(x');  v' = v_h + a' h
+// with h = dt/2.  Given the cotangents (xbar', vbar') of (x', v') and the
+// VJP (g_x, g_m) of A at x' for lam' = abar' + vbar' h, the cotangents of
+// the step's inputs are
+//   xbar = xbar' + g_x,   vbar = vbar' + xbar dt,   mbar += g_m,
+// and the cotangent of a (= A(x) from the previous step) is vbar h, so the
+// previous step's VJP weight is lam = vbar h + vbar h_prev, with h_prev = h
+// inside the trajectory and 0 for the initial acceleration.
+namespace {
+
+__global__ void k_kdk_adjoint_update(double *__restrict__ xbar, double *__restrict__ vbar,
+                                     double *__restrict__ mbar, const double *__restrict__ gx,
+                                     const double *__restrict__ gm, double *__restrict__ lam,
+                                     int n, double dt, double lam_coef)
+{
+    const int m3 = 3 * n;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m3; k += gridDim.x * blockDim.x) {
+        const double xb = xbar[k] + gx[k];
+        const double vb = fma(xb, dt, vbar[k]);
+        xbar[k] = xb;
+        vbar[k] = vb;
+        lam[k] = vb * lam_coef;
+        if (k < n) mbar[k] += gm[k];
+    }
+}
+
+}  // namespace
+
+extern "C" int rg_kdk_adjoint_update(double *xbar, double *vbar, double *mbar, const double *g_pos,
+                                     const double *g_mass, double *lam, int64_t n, double dt,
+                                     double lam_coef, void *stream)
+{
+    RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "n out of range");
+    if (n == 0) return RG_OK;
+    RG_REQUIRE(xbar && vbar && mbar && g_pos && g_mass && lam, RG_ERR_ARG, "null pointer argument");
+    const int m3 = static_cast<int>(3 * n);
+    const int grid = min((m3 + 255) / 256, 4096);
+    k_kdk_adjoint_update<<<grid, 256, 0, rg::as_stream(stream)>>>(
+        xbar, vbar, mbar, g_pos, g_mass, lam, static_cast<int>(n), dt, lam_coef);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}

@@ -54,6 +54,15 @@ SIGNATURES = {
     "rg_bh_timing": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_i64)]),
     "rg_bh_destroy": (_i, [_vp]),
     "rg_direct_vjp": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
+    "rg_kdk_adjoint_update": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _vp]),
+    "rg_batch_create": (_i, [_i64, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(_vp)]),
+    "rg_batch_set_shape": (_i, [_vp, _i, _i]),
+    "rg_batch_acc": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "rg_batch_step": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "rg_batch_graph_capture": (_i, [_vp, _vp, _vp, _vp, _vp, _i64]),
+    "rg_batch_graph_launch": (_i, [_vp, _vp]),
+    "rg_batch_energy": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "rg_batch_destroy": (_i, [_vp]),
 }
 
 _lib = None
