@@ -13,9 +13,9 @@
 //                  index tie-break) using hardware clz (:50-121)
 //   5. k_agg_*     leaf init + deterministic bottom-up aggregation: the
 //                  second child to finish combines both children (acq_rel
-//                  arrival counters; LDS and workgroup scope inside a chunk
-//                  of leaves, one workgroup for the nodes that straddle
-//                  chunks), so every node is computed exactly once and
+//                  arrival counters; LDS and workgroup scope inside each
+//                  treelet of <= 256 leaves, one workgroup for the top tree
+//                  above them), so every node is computed exactly once and
 //                  bitwise reproducibly (:124-177)
 //   6. k_walk*     stackless walk in right-first DFS order: the node
 //                  sequence the reference's stack walk pops (:180-217).
@@ -35,6 +35,9 @@
 
 #include "rg_internal.h"
 
+// Wavefront min-reduction from the ROCm device library (ockl).
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+
 namespace {
 
 constexpr int kSortBS = 256;
@@ -44,6 +47,7 @@ constexpr int kBoxBS = 256;
 constexpr int kWalkBSDefault = 256;
 
 int g_walk_bs = kWalkBSDefault;
+int g_walk_mode = 2;  // FP32 walk: 0 one lane per particle, 1 two lanes, 2 wave-coherent
 
 struct alignas(16) WalkRec64 {
     double x, y, z, m;   // centre of mass, mass
@@ -339,7 +343,7 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
                         int *__restrict__ right, int *__restrict__ parent,
                         int *__restrict__ up, int2 *__restrict__ range,
                         int *__restrict__ left_at_split, int *__restrict__ arrive,
-                        int *__restrict__ wl_count)
+                        int *__restrict__ tl_count)
 {
     const int64_t nn = n;
     for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < n - 1;
@@ -380,7 +384,7 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
         if (i == 0) {
             parent[nn] = -1;
             up[nn] = -1;
-            *wl_count = 0;
+            *tl_count = 0;
         }
     }
 }
@@ -415,6 +419,14 @@ struct NodeArrays {
     double *com, *nmass, *nmin, *nmax, *nmom;
 };
 
+// Node aggregate used when combining children: one 80-byte record.
+struct Agg {
+    double m;
+    double mom[3];
+    double mn[3];
+    double mx[3];
+};
+
 // Walk records.  FP64 (reference parity): indexed by node id, with the
 // right child and the escape (next node in right-first DFS order after the
 // subtree).  FP32: stored in right-first DFS preorder, so opening a node
@@ -430,12 +442,21 @@ struct WalkOut {
 };
 
 template <int MODE>
-__device__ __forceinline__ void write_node(const NodeArrays &na, const WalkOut &wo, int node,
-                                           int n, double m, const double mom[3],
+__device__ __forceinline__ void write_node(const NodeArrays &na, Agg *agg, const WalkOut &wo,
+                                           int node, int n, double m, const double mom[3],
                                            const double mn[3], const double mx[3],
                                            const double c[3], int lo, int hi, int right_child,
                                            int escape)
 {
+    Agg g;
+    g.m = m;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        g.mom[a] = mom[a];
+        g.mn[a] = mn[a];
+        g.mx[a] = mx[a];
+    }
+    agg[node] = g;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         na.nmom[3 * node + a] = mom[a];
@@ -466,106 +487,124 @@ __device__ __forceinline__ void write_node(const NodeArrays &na, const WalkOut &
     }
 }
 
-constexpr int kAggChunk = 256;  // leaves per workgroup in the aggregation
+constexpr int kTreelet = 256;  // max leaves of a treelet (one workgroup)
 
-// Pass 1: one thread per sorted leaf, one workgroup per chunk of kAggChunk
-// leaves.  A node whose leaf range lies inside the chunk has both children
-// finished by this workgroup: it is combined from LDS, and its arrival
-// counter is an LDS atomic at workgroup scope.  A thread that reaches a
-// node straddling chunks stops and queues that node for pass 2.
+// Treelets: the maximal subtrees with at most kTreelet leaves (a node is a
+// treelet root when its leaf count is <= kTreelet and its parent's is
+// larger).  They partition the leaves; the internal nodes above them form
+// the top tree.
+__global__ void k_classify(const int *__restrict__ parent, const int2 *__restrict__ range, int n,
+                           int *__restrict__ treelets, int *__restrict__ tl_count)
+{
+    const int m = 2 * n - 1;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m; v += gridDim.x * blockDim.x) {
+        const int size = v < n ? 1 : range[v - n].y - range[v - n].x + 1;
+        if (size > kTreelet) continue;
+        const int p = parent[v];
+        const int psize = p < 0 ? INT_MAX : range[p - n].y - range[p - n].x + 1;
+        if (psize > kTreelet) treelets[atomicAdd(tl_count, 1)] = v;
+    }
+}
+
+// Pass 1: one workgroup per treelet (grid-stride over the list), one
+// thread per leaf.  Every node of the treelet has both children inside it:
+// node data live in LDS and the arrival counters are LDS atomics at
+// workgroup scope (acq_rel: the second arriver sees the first's data).
 template <int MODE>
-__global__ __launch_bounds__(kAggChunk) void k_agg_chunk(
+__global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     const int *__restrict__ order, const double *__restrict__ pos,
     const double *__restrict__ mass, int n, const int *__restrict__ left,
     const int *__restrict__ right, const int *__restrict__ parent,
     const int2 *__restrict__ range, const int *__restrict__ left_at_split,
-    int *__restrict__ worklist, int *__restrict__ wl_count, NodeArrays na, WalkOut wo)
+    const int *__restrict__ treelets, const int *__restrict__ tl_count, NodeArrays na, Agg *agg,
+    WalkOut wo)
 {
-    constexpr int C = kAggChunk;
+    constexpr int C = kTreelet;
     __shared__ double s_m[2 * C];
     __shared__ double s_mom[3][2 * C];
     __shared__ double s_mn[3][2 * C];
     __shared__ double s_mx[3][2 * C];
     __shared__ int s_arrive[C];
-    const int c0 = blockIdx.x * C;
-    s_arrive[threadIdx.x] = 0;
-    __syncthreads();
-    const int li = c0 + threadIdx.x;
-    if (li >= n) return;  // no barrier follows
-
-    const int p = order[li];
-    const double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
-    const double m = mass[p];
-    // barnes_hut.py:129-134 (leaf init).
-    double mom[3];
-    const int sl = li - c0;
+    const int tid = threadIdx.x;
+    const int cnt = *tl_count;
+    for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
+        const int v = treelets[e];
+        const int lo = v < n ? v : range[v - n].x;
+        const int size = v < n ? 1 : range[v - n].y - lo + 1;
+        s_arrive[tid] = 0;
+        __syncthreads();
+        if (tid < size) {
+            const int li = lo + tid;
+            const int p = order[li];
+            const double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+            const double m = mass[p];
+            // barnes_hut.py:129-134 (leaf init).
+            double mom[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        mom[a] = m * x[a];
-        s_mom[a][sl] = mom[a];
-        s_mn[a][sl] = x[a];
-        s_mx[a][sl] = x[a];
-    }
-    s_m[sl] = m;
-    write_node<MODE>(na, wo, li, n, m, mom, x, x, x, li, li, -1,
-                     li == 0 ? -1 : left_at_split[li - 1]);
-
-    int node = parent[li];
-    while (node >= 0) {
-        const int ii = node - n;
-        const int2 rg = range[ii];
-        if (rg.x < c0 || rg.y >= c0 + C) {
-            worklist[atomicAdd(wl_count, 1)] = node;
-            break;
-        }
-        // acq_rel: publishes this thread's LDS/global node, acquires the sibling's.
-        if (__hip_atomic_fetch_add(&s_arrive[ii - c0], 1, __ATOMIC_ACQ_REL,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-            break;  // the sibling subtree is not finished; its thread continues
-        const int L = left[ii], R = right[ii];
-        const int sL = L < n ? L - c0 : C + (L - n - c0);
-        const int sR = R < n ? R - c0 : C + (R - n - c0);
-        const double mm = s_m[sL] + s_m[sR];
-        double mn[3], mx[3], c[3];
+            for (int a = 0; a < 3; ++a) {
+                mom[a] = m * x[a];
+                s_mom[a][tid] = mom[a];
+                s_mn[a][tid] = x[a];
+                s_mx[a][tid] = x[a];
+            }
+            s_m[tid] = m;
+            write_node<MODE>(na, agg, wo, li, n, m, mom, x, x, x, li, li, -1,
+                             li == 0 ? -1 : left_at_split[li - 1]);
+            int node = li == v ? -1 : parent[li];
+            while (node >= 0) {
+                const int ii = node - n;
+                if (__hip_atomic_fetch_add(&s_arrive[ii - lo], 1, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                    break;  // the sibling subtree is not finished; its thread continues
+                const int L = left[ii], R = right[ii];
+                const int2 rg = range[ii];
+                const int sL = L < n ? L - lo : C + (L - n - lo);
+                const int sR = R < n ? R - lo : C + (R - n - lo);
+                const double mm = s_m[sL] + s_m[sR];
+                double mn[3], mx[3], c[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            mom[a] = s_mom[a][sL] + s_mom[a][sR];
-            mn[a] = fmin(s_mn[a][sL], s_mn[a][sR]);
-            mx[a] = fmax(s_mx[a][sL], s_mx[a][sR]);
-            // barnes_hut.py:175-177: com = mom / mass where mass > 0.
-            c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
-        }
-        const int sn = C + ii - c0;
-        s_m[sn] = mm;
+                for (int a = 0; a < 3; ++a) {
+                    mom[a] = s_mom[a][sL] + s_mom[a][sR];
+                    mn[a] = fmin(s_mn[a][sL], s_mn[a][sR]);
+                    mx[a] = fmax(s_mx[a][sL], s_mx[a][sR]);
+                    // barnes_hut.py:175-177: com = mom / mass where mass > 0.
+                    c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
+                }
+                const int sn = C + ii - lo;
+                s_m[sn] = mm;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            s_mom[a][sn] = mom[a];
-            s_mn[a][sn] = mn[a];
-            s_mx[a][sn] = mx[a];
+                for (int a = 0; a < 3; ++a) {
+                    s_mom[a][sn] = mom[a];
+                    s_mn[a][sn] = mn[a];
+                    s_mx[a][sn] = mx[a];
+                }
+                write_node<MODE>(na, agg, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
+                                 rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
+                node = node == v ? -1 : parent[node];
+            }
         }
-        write_node<MODE>(na, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
-                         rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
-        node = parent[node];
+        __syncthreads();  // the LDS slots are reused by the next treelet
     }
 }
 
-// Pass 2: one workgroup finishes the nodes that straddle chunks, starting
-// from the queued arrivals.  Children finished in pass 1 are visible across
-// the kernel boundary; children finished here are published by the
-// workgroup-scope acq_rel arrival counter.
-constexpr int kUpperBS = 1024;
+// Pass 2: one workgroup finishes the top tree.  Each treelet root is an
+// arrival at its parent; the second arriver at a node combines the two
+// children's aggregates (treelet roots from pass 1, visible across the
+// kernel boundary; top nodes from this workgroup, published by the
+// workgroup-scope acq_rel arrival counter).
+constexpr int kTopBS = 1024;
 
 template <int MODE>
-__global__ __launch_bounds__(kUpperBS) void k_agg_upper(
+__global__ __launch_bounds__(kTopBS) void k_agg_top(
     int n, const int *__restrict__ left, const int *__restrict__ right,
     const int *__restrict__ parent, const int2 *__restrict__ range,
     const int *__restrict__ left_at_split, int *__restrict__ arrive,
-    const int *__restrict__ worklist, const int *__restrict__ wl_count, NodeArrays na,
+    const int *__restrict__ treelets, const int *__restrict__ tl_count, NodeArrays na, Agg *agg,
     WalkOut wo)
 {
-    const int cnt = *wl_count;
-    for (int e = threadIdx.x; e < cnt; e += kUpperBS) {
-        int node = worklist[e];
+    const int cnt = *tl_count;
+    for (int e = threadIdx.x; e < cnt; e += kTopBS) {
+        int node = parent[treelets[e]];
         while (node >= 0) {
             const int ii = node - n;
             if (__hip_atomic_fetch_add(&arrive[ii], 1, __ATOMIC_ACQ_REL,
@@ -573,16 +612,18 @@ __global__ __launch_bounds__(kUpperBS) void k_agg_upper(
                 break;
             const int L = left[ii], R = right[ii];
             const int2 rg = range[ii];
-            const double mm = na.nmass[L] + na.nmass[R];
+            const Agg aL = agg[L];
+            const Agg aR = agg[R];
+            const double mm = aL.m + aR.m;
             double mom[3], mn[3], mx[3], c[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                mom[a] = na.nmom[3 * L + a] + na.nmom[3 * R + a];
-                mn[a] = fmin(na.nmin[3 * L + a], na.nmin[3 * R + a]);
-                mx[a] = fmax(na.nmax[3 * L + a], na.nmax[3 * R + a]);
+                mom[a] = aL.mom[a] + aR.mom[a];
+                mn[a] = fmin(aL.mn[a], aR.mn[a]);
+                mx[a] = fmax(aL.mx[a], aR.mx[a]);
                 c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
             }
-            write_node<MODE>(na, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
+            write_node<MODE>(na, agg, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
                              rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
             node = parent[node];
         }
@@ -716,6 +757,170 @@ __global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict
     }
 }
 
+// FP32, two lanes per particle: the particle's own leaf slot ps splits the
+// preorder records into [0, ps) (lane 0) and [ps, 2n-1) (lane 1).  A lane
+// interacts only with the nodes it owns; a node it does not own but whose
+// subtree crosses into its range (an ancestor of the particle) is still
+// tested, and accepting it means the owning lane adds its monopole, so the
+// subtree is skipped.  The two partial sums are combined with a lane
+// shuffle.  Every node of the single-lane walk is examined by exactly one
+// owner, so the accepted set and visit counts are unchanged.
+__global__ void k_walk32_pair(const float4 *__restrict__ recA, const int2 *__restrict__ recB,
+                              const int *__restrict__ pre, const int *__restrict__ order,
+                              double *__restrict__ acc, int n, float eps2,
+                              unsigned long long *__restrict__ stats)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int g = b * blockDim.x + threadIdx.x;
+    const int t = g >> 1;
+    const int half = g & 1;
+    const bool active = t < n;
+    int visits = 0;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    if (active) {
+        const int ps = pre[t];
+        const float4 self = recA[ps];
+        const float px = self.x, py = self.y, pz = self.z;
+        const int m = 2 * n - 1;
+        const int lo = half ? ps : 0;
+        const int hi = half ? m : ps;
+        int k = 0;
+        while (k < hi) {
+            const int2 sb = recB[k];
+            const int skip = max(sb.y, k + 1);
+            if (skip <= lo) {  // subtree entirely before this lane's range
+                k = skip;
+                continue;
+            }
+            const bool own = k >= lo;
+            visits += own;
+            const float4 c = recA[k];
+            const float dx = c.x - px;
+            const float dy = c.y - py;
+            const float dz = c.z - pz;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            if (__int_as_float(sb.x) < r2) {
+                if (own) {
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float s = c.w * (inv * inv) * inv;
+                    ax = fmaf(s, dx, ax);
+                    ay = fmaf(s, dy, ay);
+                    az = fmaf(s, dz, az);
+                }
+                k = skip;
+            } else {
+                k = k + 1;
+            }
+        }
+    }
+    ax += __shfl_xor(ax, 1, 64);
+    ay += __shfl_xor(ay, 1, 64);
+    az += __shfl_xor(az, 1, 64);
+    if (active && half == 0) {
+        const int p = order[t];
+        acc[3 * p] = static_cast<double>(ax);
+        acc[3 * p + 1] = static_cast<double>(ay);
+        acc[3 * p + 2] = static_cast<double>(az);
+    }
+    const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
+    // per-particle visits = the pair's sum
+    const int pv = visits + __shfl_xor(visits, 1, 64);
+    const int mx = wave_max_i32(pv);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], tot);
+        atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+    }
+}
+
+// FP32, one wave walks together: the wave steps through the preorder
+// records in increasing slot order, visiting every slot that at least one
+// of its 64 (Morton-adjacent) particles wants next; a lane whose next wanted
+// slot is the current one tests it, the others wait.  The per-lane accepted
+// set, visit count and summation order are exactly the single-lane walk's.
+// Because the slots only increase, the wave keeps a window of 64
+// consecutive records in registers (lane l holds slot k0 + l), refilled by
+// one coalesced load when the walk leaves it; the current record is read
+// with v_readlane into scalar registers, so most steps touch no memory.
+__global__ __launch_bounds__(256) void k_walk32_wave(const float4 *__restrict__ recA,
+                                                     const int2 *__restrict__ recB,
+                                                     const int *__restrict__ pre,
+                                                     const int *__restrict__ order,
+                                                     double *__restrict__ acc, int n, float eps2,
+                                                     unsigned long long *__restrict__ stats)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int t = b * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool active = t < n;
+    const int m = 2 * n - 1;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (active) {
+        const float4 self = recA[pre[t]];
+        px = self.x;
+        py = self.y;
+        pz = self.z;
+    }
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    int visits = 0;
+    int next = active ? 0 : m;
+    int k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+    int k0 = -64;  // window start (no window yet)
+    float4 wa = make_float4(0.f, 0.f, 0.f, 0.f);
+    int2 wb = make_int2(0, 0);
+    while (k < m) {
+        if (k - k0 >= 64) {
+            k0 = k;
+            const int slot = k0 + lane;
+            if (slot < m) {
+                wa = recA[slot];
+                wb = recB[slot];
+            }
+        }
+        const int l = k - k0;
+        const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
+        const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
+        const float cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
+        const float cw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
+        const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+        const int skip = __builtin_amdgcn_readlane(wb.y, l);
+        if (next == k) {
+            ++visits;
+            const float dx = cx - px;
+            const float dy = cy - py;
+            const float dz = cz - pz;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            if (crit < r2) {
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float s = cw * (inv * inv) * inv;
+                ax = fmaf(s, dx, ax);
+                ay = fmaf(s, dy, ay);
+                az = fmaf(s, dz, az);
+                next = max(skip, k + 1);
+            } else {
+                next = k + 1;
+            }
+        }
+        // every lane's next slot is > k; the smallest is k + 1 whenever a
+        // lane opened node k
+        if (__ballot(next == k + 1))
+            k = k + 1;
+        else
+            k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+    }
+    if (active) {
+        const int p = order[t];
+        acc[3 * p] = static_cast<double>(ax);
+        acc[3 * p + 1] = static_cast<double>(ay);
+        acc[3 * p + 2] = static_cast<double>(az);
+    }
+    const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
+    const int mx = wave_max_i32(visits);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], tot);
+        atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+    }
+}
+
 }  // namespace
 
 // ================================================================ handle
@@ -737,7 +942,8 @@ struct rg_bh {
     int *left = nullptr, *right = nullptr, *parent = nullptr;
     int2 *range = nullptr;
     int *left_at_split = nullptr, *arrive = nullptr;
-    int *up = nullptr, *pre = nullptr, *worklist = nullptr, *wl_count = nullptr;
+    int *up = nullptr, *pre = nullptr, *treelets = nullptr, *tl_count = nullptr;
+    Agg *agg = nullptr;
     NodeArrays na{};
     WalkRec64 *rec64 = nullptr;  // FP64 walk records (node-id order)
     float4 *recA = nullptr;      // FP32 walk records (preorder)
@@ -779,6 +985,13 @@ int bh_mark(rg_bh *h, int phase, hipStream_t st)
 }
 
 }  // namespace
+
+int rg::rg_bh_set_walk_mode_impl(int64_t v)
+{
+    RG_REQUIRE(v >= 0 && v <= 2, RG_ERR_ARG, "walk_mode must be 0, 1 or 2");
+    g_walk_mode = static_cast<int>(v);
+    return RG_OK;
+}
 
 int rg::rg_bh_set_walk_block_impl(int64_t v)
 {
@@ -846,8 +1059,9 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->na.nmax, 24 * M);
     ALLOC(h->na.nmom, 24 * M);
     ALLOC(h->up, 4 * M);
-    ALLOC(h->worklist, 4 * 2 * N);
-    ALLOC(h->wl_count, 16);
+    ALLOC(h->treelets, 4 * M);
+    ALLOC(h->tl_count, 16);
+    ALLOC(h->agg, sizeof(Agg) * M);
     if (precision == RG_FP64) {
         ALLOC(h->rec64, sizeof(WalkRec64) * M);
     } else {
@@ -906,9 +1120,14 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
 
     const int g1 = min((n + 255) / 256, 16384);
     k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->up, h->range,
-                                h->left_at_split, h->arrive, h->wl_count);
+                                h->left_at_split, h->arrive, h->tl_count);
     RG_CHECK_LAUNCH();
     const bool fp64 = h->precision == RG_FP64;
+    {
+        const int gc = min((2 * n - 1 + 255) / 256, 16384);
+        k_classify<<<gc, 256, 0, st>>>(h->parent, h->range, n, h->treelets, h->tl_count);
+        RG_CHECK_LAUNCH();
+    }
     if (!fp64) {
         const int gp = min((2 * n - 1 + 255) / 256, 16384);
         k_preorder<<<gp, 256, 0, st>>>(h->up, h->range, n, h->pre, h->stats);
@@ -926,23 +1145,23 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
     wo.inv_theta2 = wo.theta_zero ? 0.0 : 1.0 / theta2;
     rc = bh_mark(h, 2, st);
     if (rc != RG_OK) return rc;
-    const int ga = (n + kAggChunk - 1) / kAggChunk;
+    const int gt = min(n, 4096);
     if (fp64) {
-        k_agg_chunk<0><<<ga, kAggChunk, 0, st>>>(order, pos, mass, n, h->left, h->right,
-                                                h->parent, h->range, h->left_at_split,
-                                                h->worklist, h->wl_count, h->na, wo);
+        k_agg_treelet<0><<<gt, kTreelet, 0, st>>>(order, pos, mass, n, h->left, h->right,
+                                                 h->parent, h->range, h->left_at_split,
+                                                 h->treelets, h->tl_count, h->na, h->agg, wo);
         RG_CHECK_LAUNCH();
-        k_agg_upper<0><<<1, kUpperBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,
-                                               h->left_at_split, h->arrive, h->worklist,
-                                               h->wl_count, h->na, wo);
+        k_agg_top<0><<<1, kTopBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,
+                                           h->left_at_split, h->arrive, h->treelets,
+                                           h->tl_count, h->na, h->agg, wo);
     } else {
-        k_agg_chunk<1><<<ga, kAggChunk, 0, st>>>(order, pos, mass, n, h->left, h->right,
-                                                h->parent, h->range, h->left_at_split,
-                                                h->worklist, h->wl_count, h->na, wo);
+        k_agg_treelet<1><<<gt, kTreelet, 0, st>>>(order, pos, mass, n, h->left, h->right,
+                                                 h->parent, h->range, h->left_at_split,
+                                                 h->treelets, h->tl_count, h->na, h->agg, wo);
         RG_CHECK_LAUNCH();
-        k_agg_upper<1><<<1, kUpperBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,
-                                               h->left_at_split, h->arrive, h->worklist,
-                                               h->wl_count, h->na, wo);
+        k_agg_top<1><<<1, kTopBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,
+                                           h->left_at_split, h->arrive, h->treelets,
+                                           h->tl_count, h->na, h->agg, wo);
     }
     RG_CHECK_LAUNCH();
     rc = bh_mark(h, 3, st);
@@ -954,8 +1173,18 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
     if (fp64) {
         k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, acc, n, h->G, theta2, eps2, h->stats);
     } else {
-        k_walk32<<<wg, wb, 0, st>>>(h->recA, h->recB, h->pre, order, acc, n,
-                                    static_cast<float>(eps2), h->stats);
+        if (g_walk_mode == 2) {
+            k_walk32_wave<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
+                                                           acc, n, static_cast<float>(eps2),
+                                                           h->stats);
+        } else if (g_walk_mode == 1) {
+            const int wg2 = (2 * n + wb - 1) / wb;
+            k_walk32_pair<<<wg2, wb, 0, st>>>(h->recA, h->recB, h->pre, order, acc, n,
+                                              static_cast<float>(eps2), h->stats);
+        } else {
+            k_walk32<<<wg, wb, 0, st>>>(h->recA, h->recB, h->pre, order, acc, n,
+                                        static_cast<float>(eps2), h->stats);
+        }
     }
     RG_CHECK_LAUNCH();
     rc = bh_mark(h, 4, st);

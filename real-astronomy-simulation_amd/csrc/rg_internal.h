@@ -22,6 +22,7 @@ int device_scratch(size_t bytes, void **ptr);
 // Tunables behind rg_set_param.
 int rg_direct_set_ipt_impl(int64_t v);
 int rg_bh_set_walk_block_impl(int64_t v);
+int rg_bh_set_walk_mode_impl(int64_t v);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

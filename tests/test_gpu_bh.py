@@ -172,3 +172,26 @@ def test_fp32_walk_visits_match_reference_walk(orc, name):
     _, visits, ovf = orc.bh_acc(pos, mass, 1.0, 0.5, 0.05, max_stack=4096)
     st = bh.stats()
     assert abs(st["visits_total"] - visits.sum()) <= 1e-3 * visits.sum() + 2
+
+
+@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3"])
+def test_fp32_walk_modes_agree(name):
+    """The two-lane and wave-coherent walks examine the same nodes as the
+    one-lane walk; the two-lane walk adds two FP32 partial sums at the end,
+    the wave walk keeps each lane's summation order (bitwise equal)."""
+    from rgsim import _lib
+    pos, mass = CASES[name]
+    res = {}
+    for mode in (0, 1, 2):
+        _lib.set_param("walk_mode", mode)
+        try:
+            bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
+            res[mode] = (acc, bh.stats())
+        finally:
+            _lib.set_param("walk_mode", 2)
+    (a0, s0), (a1, s1), (a2, s2) = res[0], res[1], res[2]
+    for s in (s1, s2):
+        assert s0["visits_total"] == s["visits_total"] and s0["visits_max"] == s["visits_max"]
+    scale = np.linalg.norm(a0, axis=1).max()
+    assert np.abs(a0 - a1).max() <= 1e-5 * scale
+    assert np.array_equal(a0, a2)
