@@ -101,10 +101,11 @@ def bench_bh(steps, warmup):
     }
 
 
-def bench_direct(n=262144, evals=3):
+def bench_direct(n=262144, evals=3, kdk_steps=50):
     import numpy as np
     import torch
 
+    from rgsim.engine import Engine
     from rgsim.ic import make_plummer
     from rgsim.solvers import gravity
 
@@ -112,7 +113,7 @@ def bench_direct(n=262144, evals=3):
     pos = torch.from_numpy(s.pos).cuda()
     mass = torch.from_numpy(s.mass).cuda()
     eps2 = 0.005 ** 2
-    out = {}
+    out = {"n": n}
     ref = None
     for prec in ("fp64", "fp32", "fp32c"):
         acc = torch.empty_like(pos)
@@ -131,14 +132,192 @@ def bench_direct(n=262144, evals=3):
         gint = n * n / (ms * 1e-3) / 1e9
         peak = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
         tflops = gint * FLOP_PER_INTERACTION / 1e3
-        out[prec] = {"ms_per_eval": ms, "Gint_per_s": gint, "TFLOP_s_20flop": tflops,
-                     "frac_of_peak": tflops / peak,
-                     "force_err_vs_fp64": rel_err(a, ref) if prec != "fp64" else None}
+        entry = {"ms_per_eval": ms, "Gint_per_s": gint, "TFLOP_s_20flop": tflops,
+                 "frac_of_peak": tflops / peak}
+        if prec != "fp64":
+            entry["force_err_vs_fp64"] = rel_err(a, ref)
+        out[prec] = entry
         log(f"direct {prec}: {ms:.2f} ms/eval, {gint:.1f} Gint/s")
     out["speedup_fp32c_vs_fp64"] = out["fp64"]["ms_per_eval"] / out["fp32c"]["ms_per_eval"]
     out["speedup_fp32_vs_fp64"] = out["fp64"]["ms_per_eval"] / out["fp32"]["ms_per_eval"]
-    out["n"] = n
+    # precision form: 50 KDK steps dt 5e-4, FP64 energy and momentum
+    eng = Engine(softening=0.005, precision="fp32c", G=1.0)
+    eng.load_state(s)
+    ke0, pe0, e_0 = eng.energies()
+    p0 = np.array(eng.momentum())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(kdk_steps):
+        eng.step(5e-4)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ke1, pe1, e_1 = eng.energies()
+    p1 = np.array(eng.momentum())
+    out["fp32c_kdk"] = {"steps": kdk_steps, "dt": 5e-4, "ms_per_step": 1e3 * wall / kdk_steps,
+                        "energy_drift": abs((e_1 - e_0) / e_0), "virial": -2 * ke1 / pe1,
+                        "momentum_change": float(np.linalg.norm(p1 - p0))}
+    log(f"direct fp32c KDK: {out['fp32c_kdk']['ms_per_step']:.2f} ms/step, "
+        f"|dE/E| {out['fp32c_kdk']['energy_drift']:.2e}")
     return out
+
+
+def bench_adjoint(n=16384, steps=400, ckpt=20):
+    """Differentiation form: forward KDK step vs adjoint (reverse) step."""
+    import torch
+
+    from rgsim.adjoint import direct_vjp, kdk_trajectory
+    from rgsim.ic import make_plummer
+    from rgsim.solvers import gravity
+
+    s = make_plummer(n, total_mass=1.0, scale_radius=1.0, seed=7, G=1.0)
+    x = torch.from_numpy(s.pos).cuda()
+    m = torch.from_numpy(s.mass).cuda()
+    lam = torch.randn_like(x)
+    eps2 = 0.01 ** 2
+    acc = torch.empty_like(x)
+    reps = 20
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {"n": n, "steps": steps, "checkpoint_every": ckpt, "precision": "fp32c"}
+    out["force_ms"] = timed(lambda: gravity.compute_acc_direct(x, m, acc, n, 1.0, eps2, "fp32c"))
+    out["vjp_ms"] = timed(lambda: direct_vjp(x, m, lam, 1.0, eps2, "fp32c"))
+    out["vjp_over_force"] = out["vjp_ms"] / out["force_ms"]
+    x0 = torch.from_numpy(s.pos).cuda().requires_grad_(True)
+    v0 = torch.from_numpy(s.vel).cuda().requires_grad_(True)
+    mm = torch.from_numpy(s.mass).cuda().requires_grad_(True)
+    target = x0.detach() + 1e-3 * torch.randn_like(x0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xN, _ = kdk_trajectory(x0, v0, mm, steps, 1e-3, 1.0, 0.01, ckpt, "fp32c")
+    loss = ((xN - target) ** 2).sum(1).mean()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out["forward_ms_per_step"] = 1e3 * (t1 - t0) / steps
+    out["backward_ms_per_step"] = 1e3 * (t2 - t1) / steps
+    out["backward_over_forward"] = (t2 - t1) / (t1 - t0)
+    out["backward_includes"] = "segment recompute (1 force/step) + 1 VJP/step + update"
+    log(f"adjoint: force {out['force_ms']:.3f} ms, vjp {out['vjp_ms']:.3f} ms "
+        f"({out['vjp_over_force']:.2f}x); trajectory bwd/fwd {out['backward_over_forward']:.2f}x")
+    return out
+
+
+def bench_batch(steps=100):
+    """Batching form: 512 Plummer systems, N log-uniform in [256, 16384]."""
+    import numpy as np
+    import torch
+
+    from rgsim.autotune import tune_batch
+    from rgsim.batch import Ensemble, plummer_ensemble
+
+    systems = plummer_ensemble(nsys=512, n_min=256, n_max=16384, master_seed=2026)
+    ens = Ensemble(systems, precision="fp32c")
+    tune = tune_batch(ens, reps=2)
+    e0 = ens.energies()
+    ens.capture(steps)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    ens.replay()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    e1 = ens.energies()
+    inter = ens.interactions_per_step()
+    # one-launch-per-system baseline, same kernels
+    ps_steps = 5
+    ens2 = Ensemble(systems, precision="fp32c")
+    ens2.per_system_step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens2.per_system_step(ps_steps)
+    torch.cuda.synchronize()
+    ps_ms = 1e3 * (time.perf_counter() - t0) / ps_steps
+    # FP64 batched run of the same steps for the drift comparison
+    ens64 = Ensemble(systems, precision="fp64")
+    f0 = ens64.energies()
+    ens64.capture(steps)
+    ens64.replay()
+    f1 = ens64.energies()
+    d32 = np.abs((e1[:, 0] + e1[:, 1] - e0[:, 0] - e0[:, 1]) / (e0[:, 0] + e0[:, 1]))
+    d64 = np.abs((f1[:, 0] + f1[:, 1] - f0[:, 0] - f0[:, 1]) / (f0[:, 0] + f0[:, 1]))
+    out = {"nsys": ens.nsys, "ntot": ens.ntot, "interactions_per_step": inter,
+           "ms_per_step_graph": ms, "Gint_per_s_graph": inter / (ms * 1e-3) / 1e9,
+           "ms_per_step_per_system_launches": ps_ms,
+           "Gint_per_s_per_system_launches": inter / (ps_ms * 1e-3) / 1e9,
+           "speedup_vs_per_system": ps_ms / ms,
+           "autotune": {k: v for k, v in tune.items() if k != "trials"},
+           "energy_drift_fp32c": {"max": float(d32.max()), "median": float(np.median(d32))},
+           "energy_drift_fp64": {"max": float(d64.max()), "median": float(np.median(d64))}}
+    log(f"batch: {out['Gint_per_s_graph']:.0f} Gint/s graph vs "
+        f"{out['Gint_per_s_per_system_launches']:.0f} per-system; drift fp32c max {d32.max():.1e}")
+    return out
+
+
+def bench_bh_1m(steps=5):
+    """BH at N=1,000,000: disk:bulge 5:1, theta 0.5, FP32 walk."""
+    import torch
+
+    from rgsim.engine import Engine
+    from rgsim.ic import make_disk_bulge
+    from rgsim.solvers import gravity
+
+    s = make_disk_bulge(n_disk=833334, n_bulge=166666, seed=1)
+    n = s.n
+    eng = Engine(softening=0.05, gravity_mode="bh", theta=0.5, precision="fp32")
+    eng.load_state(s)
+    ref = torch.empty_like(eng.pos)
+    gravity.compute_acc_direct(eng.pos, eng.mass, ref, n, eng.G, 0.05 ** 2, "fp32c")
+    err = rel_err(eng.acc.cpu().numpy(), ref.cpu().numpy())
+    eng.step(1e-4)
+    torch.cuda.synchronize()
+    eng._bh.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.step(1e-4)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tm = eng._bh.timing()
+    eng._bh.set_timing(False)
+    st = eng._bh.stats()
+    c = max(tm["count"], 1)
+    walk_ms = tm["walk_ms"] / c
+    walk_bytes = st["visits_total"] * WALK_RECORD_BYTES + n * WALK_PARTICLE_BYTES
+    out = {"n": n, "ms_per_step": 1e3 * wall / steps, "mps": n * steps / wall / 1e6,
+           "phase_ms": {"sort": tm["sort_ms"] / c, "build": tm["build_ms"] / c,
+                        "aggregate": tm["aggregate_ms"] / c, "walk": walk_ms},
+           "visits_per_particle": st["visits_total"] / n,
+           "walk_GBs_algorithmic": walk_bytes / (walk_ms * 1e-3) / 1e9,
+           "walk_frac_of_hbm_peak": walk_bytes / (walk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "force_err_vs_direct": err, "force_err_reference": "FP32-compensated direct sum"}
+    log(f"bh 1M: {out['ms_per_step']:.2f} ms/step, walk {walk_ms:.2f} ms "
+        f"({out['walk_frac_of_hbm_peak']:.2f} of HBM peak)")
+    return out
+
+
+def copy_calibration(mib=1024, reps=3):
+    """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    a = torch.empty(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": a.numel() * 4,
+                      "reps": reps}))
 
 
 def cpu_baseline(steps=2):
@@ -176,6 +355,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the BH drop-in workload (no direct/adjoint/batch/1M sections)")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -186,14 +368,24 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     log("bench: BH disk+bulge N=120000 ...")
     bh = bench_bh(args.steps, args.warmup)
     log(f"bench: BH {bh['mps']:.2f} Mparticle-steps/s, {bh['ms_per_step']:.3f} ms/step")
-    direct = None
-    if not args.no_direct:
+    direct = adjoint = batch = bh1m = None
+    if not args.no_direct and not args.no_extra:
         log("bench: direct Plummer N=262144 ...")
         direct = bench_direct()
+    if not args.no_extra:
+        log("bench: adjoint Plummer N=16384 ...")
+        adjoint = bench_adjoint()
+        log("bench: batch 512 systems ...")
+        batch = bench_batch()
+        log("bench: BH N=1M ...")
+        bh1m = bench_bh_1m()
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")
@@ -222,6 +414,9 @@ def main():
         "cpu_baseline": cpu,
         "bh": {k: v for k, v in bh.items() if k != "roofline"},
         "direct_plummer_262144": direct,
+        "adjoint_plummer_16384": adjoint,
+        "batch_512_systems": batch,
+        "bh_disk_bulge_1M": bh1m,
     }
     print(json.dumps(line))
 

@@ -25,6 +25,7 @@ constexpr int kBS = 256;      // threads per block = j-tile length
 constexpr int kChunk = 32;    // FP32C: pair terms per partial before the two-sum fold
 
 int g_direct_ipt = 0;         // 0 = heuristic; else i-particles per thread (1, 2, 4)
+int g_fp32c_d64 = 0;          // FP32C pair difference: 0 hi/lo FP32, 1 FP64 then rounded
 
 __device__ __forceinline__ void split_df(double x, float &hi, float &lo)
 {
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
                                                 double *__restrict__ vel, int n, double G,
                                                 double eps2, double dt_half)
 {
-    using T4 = typename std::conditional<MODE == 0, double4, float4>::type;
+    using T4 = typename std::conditional<MODE == 0 || MODE == 3, double4, float4>::type;
     __shared__ T4 s_hi[kBS];
     __shared__ float4 s_lo[MODE == 2 ? kBS : 1];
 
@@ -87,6 +88,8 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
             const double qm = mass[j];
             if constexpr (MODE == 0) {
                 s_hi[tid] = make_double4(qx, qy, qz, qm);
+            } else if constexpr (MODE == 3) {
+                s_hi[tid] = make_double4(qx, qy, qz, G * qm);
             } else if constexpr (MODE == 1) {
                 s_hi[tid] = make_float4(static_cast<float>(qx), static_cast<float>(qy),
                                         static_cast<float>(qz), static_cast<float>(G * qm));
@@ -133,6 +136,41 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
                     sx[k] = fmaf(s, dx, sx[k]);
                     sy[k] = fmaf(s, dy, sy[k]);
                     sz[k] = fmaf(s, dz, sz[k]);
+                }
+            }
+        } else if constexpr (MODE == 3) {
+            for (int c0 = 0; c0 < cnt; c0 += kChunk) {
+                const int c1 = min(c0 + kChunk, cnt);
+                float px[IPT], py[IPT], pz[IPT];
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
+#pragma unroll 4
+                for (int jj = c0; jj < c1; ++jj) {
+                    const double4 q = s_hi[jj];
+                    const float gm = static_cast<float>(q.w);
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) {
+                        // FP64 difference (full rate), rounded once to FP32
+                        const float dx = static_cast<float>(q.x - xd[k]);
+                        const float dy = static_cast<float>(q.y - yd[k]);
+                        const float dz = static_cast<float>(q.z - zd[k]);
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        const float s = gm * (inv * inv) * inv;
+                        px[k] = fmaf(s, dx, px[k]);
+                        py[k] = fmaf(s, dy, py[k]);
+                        pz[k] = fmaf(s, dz, pz[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) {
+                    float e;
+                    two_sum(sx[k], px[k], sx[k], e);
+                    cx[k] += e;
+                    two_sum(sy[k], py[k], sy[k], e);
+                    cy[k] += e;
+                    two_sum(sz[k], pz[k], sz[k], e);
+                    cz[k] += e;
                 }
             }
         } else {
@@ -202,16 +240,9 @@ template <int MODE>
 int launch_direct(const double *pos, const double *mass, double *acc, double *vel, int n,
                   double G, double eps2, double dt_half, hipStream_t st)
 {
-    int ipt = g_direct_ipt;
-    if (ipt == 0) {
-        // Keep at least ~2 blocks per CU (256 CUs) in flight; otherwise give
-        // each lane more i-particles to amortise the LDS broadcast reads.
-        ipt = 1;
-        if (MODE != 0) {
-            if (n >= 256 * kBS * 4 * 2) ipt = 4;
-            else if (n >= 256 * kBS * 2 * 2) ipt = 2;
-        }
-    }
+    // One i-particle per lane measured fastest in every precision (the
+    // kernels are VALU-bound; more i per lane only adds register pressure).
+    const int ipt = g_direct_ipt == 0 ? 1 : g_direct_ipt;
     const int per_block = kBS * ipt;
     const int grid = (n + per_block - 1) / per_block;
     switch (ipt) {
@@ -254,7 +285,9 @@ int direct_common(const double *pos, const double *mass, double *acc, double *ve
     switch (precision) {
     case RG_FP64: return launch_direct<0>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
     case RG_FP32: return launch_direct<1>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
-    case RG_FP32C: return launch_direct<2>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
+    case RG_FP32C:
+        if (g_fp32c_d64) return launch_direct<3>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
+        return launch_direct<2>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
     default: break;
     }
     rg::set_error("unknown precision %d", precision);
@@ -450,5 +483,12 @@ extern "C" int rg_energy(const double *pos, const double *vel, const double *mas
     RG_CHECK_LAUNCH();
     k_energy_final<<<1, kEB, 0, s>>>(part, nb, out5);
     RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
+int rg::rg_direct_set_fp32c_d64_impl(int64_t v)
+{
+    RG_REQUIRE(v == 0 || v == 1, RG_ERR_ARG, "fp32c_d64 must be 0 or 1");
+    g_fp32c_d64 = static_cast<int>(v);
     return RG_OK;
 }

@@ -21,6 +21,7 @@ int device_scratch(size_t bytes, void **ptr);
 
 // Tunables behind rg_set_param.
 int rg_direct_set_ipt_impl(int64_t v);
+int rg_direct_set_fp32c_d64_impl(int64_t v);
 int rg_bh_set_walk_block_impl(int64_t v);
 int rg_bh_set_walk_mode_impl(int64_t v);
 

@@ -158,3 +158,55 @@ def test_gpu_trajectory_vs_central_differences():
         fd = (vals[0] - vals[1]) / (2 * h)
         g = grads[which][idx]
         assert abs(g - fd) <= 1e-5 * abs(fd) + 1e-9, (which, idx, g, fd)
+
+
+@pytest.mark.gpu
+def test_gpu_full_config_central_differences():
+    """Differentiation config at full size: Plummer N=16384 (eps 0.01), 400
+    KDK steps dt 1e-3, checkpoints every 20 steps, loss = mean squared
+    distance of the final positions to a perturbed target; FP64 central
+    differences (h = 1e-6) on 64 random coordinates of (x0, v0, m).  The FP64
+    adjoint must match to 1e-6 relative; the FP32-compensated adjoint's
+    deviation is bounded at 1e-3 relative of the gradient scale."""
+    from rgsim.adjoint import kdk_trajectory
+    from rgsim.ic import make_plummer
+    print("full adjoint config N=16384 steps=400", flush=True)
+    s = make_plummer(16384, total_mass=1.0, scale_radius=1.0, seed=7, G=1.0)
+    rng = np.random.default_rng(9)
+    tgt_state = make_plummer(16384, total_mass=1.0, scale_radius=1.0, seed=7, G=1.0)
+    tgt0 = tgt_state.pos + 1e-3 * np.random.default_rng(8).normal(size=s.pos.shape)
+    with torch.no_grad():
+        tgt, _ = kdk_trajectory(torch.tensor(tgt0, device="cuda"),
+                                torch.tensor(tgt_state.vel, device="cuda"),
+                                torch.tensor(tgt_state.mass, device="cuda"),
+                                400, 1e-3, 1.0, 0.01, 20, "fp64")
+    base = {"x": s.pos, "v": s.vel, "m": s.mass}
+
+    def loss_of(arrs, precision="fp64"):
+        xN, _ = kdk_trajectory(*arrs, 400, 1e-3, 1.0, 0.01, 20, precision)
+        return ((xN - tgt) ** 2).sum(1).mean()
+
+    grads = {}
+    for precision in ("fp64", "fp32c"):
+        ts = [torch.tensor(base[k], device="cuda", requires_grad=True) for k in "xvm"]
+        loss_of(ts, precision).backward()
+        grads[precision] = {k: t.grad.cpu().numpy() for k, t in zip("xvm", ts)}
+    h = 1e-6
+    worst64 = worst32 = 0.0
+    for c in range(64):
+        which = "xvm"[c % 3]
+        arr = base[which]
+        idx = tuple(int(rng.integers(0, d)) for d in arr.shape)
+        vals = []
+        for sgn in (1, -1):
+            a = {k: v.copy() for k, v in base.items()}
+            a[which][idx] += sgn * h
+            with torch.no_grad():
+                vals.append(loss_of([torch.tensor(a[k], device="cuda") for k in "xvm"]).item())
+        fd = (vals[0] - vals[1]) / (2 * h)
+        scale = np.abs(grads["fp64"][which]).max()
+        worst64 = max(worst64, abs(grads["fp64"][which][idx] - fd) / scale)
+        worst32 = max(worst32, abs(grads["fp32c"][which][idx] - fd) / scale)
+    print(f"max |adjoint - FD| / max|grad|: fp64 {worst64:.2e}, fp32c {worst32:.2e}")
+    assert worst64 < 1e-6
+    assert worst32 < 1e-3
