@@ -7,7 +7,8 @@
 //
 //   1. k_bbox_*    FP64 bounding box (exact min/max, fixed-order reduction)
 //   2. k_morton    keys bit-identical to morton_codes (barnes_hut.py:38-46)
-//   3. k_radix_*   stable LSD radix sort of (key, index), 8-bit digits; the
+//   3. k_onesweep  stable LSD radix sort of (key, index), 8-bit digits, one
+//                  kernel per digit with decoupled look-back; the
 //                  permutation equals np.argsort(kind="stable") (:247)
 //   4. k_build     Karras build with the reference's exact delta (incl. the
 //                  index tie-break) using hardware clz (:50-121)
@@ -41,8 +42,9 @@ extern "C" __device__ int __ockl_wfred_min_i32(int);
 namespace {
 
 constexpr int kSortBS = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortBS * kSortItems;  // keys per sort block
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortBS * kSortItems;  // keys per sort tile
+constexpr int kMaxPasses = 8;                    // 8-bit digits of a 64-bit key
 constexpr int kBoxBS = 256;
 constexpr int kWalkBSDefault = 256;
 
@@ -111,8 +113,13 @@ __global__ __launch_bounds__(kBoxBS) void k_bbox_partial(const double *__restric
 // box = {lo_x, lo_y, lo_z, scale}; stats[0..1] reset, stats[2] = non-finite.
 __global__ __launch_bounds__(kBoxBS) void k_bbox_final(const double *__restrict__ part, int nb,
                                                         int bits, double *__restrict__ box,
-                                                        unsigned long long *__restrict__ stats)
+                                                        unsigned long long *__restrict__ stats,
+                                                        unsigned *__restrict__ ghist,
+                                                        int *__restrict__ tickets)
 {
+    if (ghist)  // the sort's global digit histograms and tile tickets start at zero
+        for (int k = threadIdx.x; k < kMaxPasses * 256; k += kBoxBS) ghist[k] = 0u;
+    if (tickets && threadIdx.x < kMaxPasses) tickets[threadIdx.x] = 0;
     __shared__ double s[7][kBoxBS];
     const int t = threadIdx.x;
     double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
@@ -174,9 +181,17 @@ __device__ __forceinline__ uint64_t part1by2(uint64_t x)
     return x;
 }
 
-__global__ void k_morton(const double *__restrict__ pos, int n, int bits,
-                         const double *__restrict__ box, uint64_t *__restrict__ keys)
+// Keys, plus (when passes > 0) the global histogram of every 8-bit digit
+// for the radix sort and the reset of the sort's look-back flags.
+__global__ __launch_bounds__(256) void k_morton(const double *__restrict__ pos, int n, int bits,
+                                                const double *__restrict__ box,
+                                                uint64_t *__restrict__ keys, int passes,
+                                                unsigned *__restrict__ ghist,
+                                                unsigned *__restrict__ lookback, int lb_words)
 {
+    __shared__ unsigned s_h[kMaxPasses * 256];
+    for (int k = threadIdx.x; k < passes * 256; k += blockDim.x) s_h[k] = 0u;
+    __syncthreads();
     const double lo0 = box[0], lo1 = box[1], lo2 = box[2], scale = box[3];
     const double qmax = static_cast<double>((1ull << bits) - 1ull);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -191,19 +206,26 @@ __global__ void k_morton(const double *__restrict__ pos, int n, int bits,
             key |= part1by2(static_cast<uint64_t>(t)) << a;
         }
         keys[i] = key;
+        for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p * 256 + ((key >> (8 * p)) & 0xFFu)], 1u);
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < passes * 256; k += blockDim.x)
+        if (s_h[k]) atomicAdd(&ghist[k], s_h[k]);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < lb_words; k += gridDim.x * blockDim.x)
+        lookback[k] = 0u;
 }
 
 int launch_bbox_morton(const double *pos, int n, int bits, double *part, double *box,
-                       unsigned long long *stats, uint64_t *keys, hipStream_t st)
+                       unsigned long long *stats, uint64_t *keys, int passes, unsigned *ghist,
+                       int *tickets, unsigned *lookback, int lb_words, hipStream_t st)
 {
     const int nb = min((n + kBoxBS - 1) / kBoxBS, 1024);
     k_bbox_partial<<<nb, kBoxBS, 0, st>>>(pos, n, part);
     RG_CHECK_LAUNCH();
-    k_bbox_final<<<1, kBoxBS, 0, st>>>(part, nb, bits, box, stats);
+    k_bbox_final<<<1, kBoxBS, 0, st>>>(part, nb, bits, box, stats, ghist, tickets);
     RG_CHECK_LAUNCH();
-    const int mg = min((n + 255) / 256, 8192);
-    k_morton<<<mg, 256, 0, st>>>(pos, n, bits, box, keys);
+    const int mg = min((n + 255) / 256, 2048);
+    k_morton<<<mg, 256, 0, st>>>(pos, n, bits, box, keys, passes, ghist, lookback, lb_words);
     RG_CHECK_LAUNCH();
     return RG_OK;
 }
@@ -215,67 +237,49 @@ size_t bbox_part_doubles(int n)
 }
 
 // --------------------------------------------------------- radix sort
-// Pass: hist -> exclusive scan (digit-major, block-minor) -> stable scatter.
-__global__ __launch_bounds__(kSortBS) void k_radix_hist(const uint64_t *__restrict__ keys, int n,
-                                                         int shift, int nblk,
-                                                         unsigned *__restrict__ hist)
-{
-    __shared__ unsigned s_h[256];
-    const int t = threadIdx.x;
-    s_h[t] = 0u;
-    __syncthreads();
-    const int base = blockIdx.x * kSortTile;
-#pragma unroll 4
-    for (int r = 0; r < kSortItems; ++r) {
-        const int i = base + r * kSortBS + t;
-        if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 0xFFu], 1u);
-    }
-    __syncthreads();
-    hist[t * nblk + blockIdx.x] = s_h[t];
-}
+// Stable LSD radix sort, one kernel per 8-bit digit ("one sweep"): each
+// tile of kSortTile keys takes a ticket (tiles start in ticket order), counts
+// its digits, publishes the counts and looks back over earlier tiles'
+// published counts (decoupled look-back) for its exclusive offsets.  A
+// look-back word packs a status (1 = this tile's count, 2 = inclusive
+// prefix) with a 30-bit count, so a single relaxed agent-scope atomic
+// carries both.  Every wait is on a tile that started earlier and publishes
+// without waiting, and is bounded; exceeding the bound flags stats[3]
+// instead of hanging.
+constexpr unsigned kFlagAgg = 1u << 30;
+constexpr unsigned kFlagInc = 2u << 30;
+constexpr unsigned kCountMask = (1u << 30) - 1u;
+constexpr int kSpinLimit = 1 << 20;
 
-constexpr int kScanBS = 1024;
-
-__global__ __launch_bounds__(kScanBS) void k_scan_exclusive(unsigned *__restrict__ a, int m)
-{
-    __shared__ unsigned s[kScanBS];
-    const int t = threadIdx.x;
-    const int per = (m + kScanBS - 1) / kScanBS;
-    const int b0 = t * per;
-    const int b1 = min(b0 + per, m);
-    unsigned sum = 0;
-    for (int k = b0; k < b1; ++k) sum += a[k];
-    s[t] = sum;
-    __syncthreads();
-    // Hillis-Steele inclusive scan of the per-thread sums.
-    for (int off = 1; off < kScanBS; off <<= 1) {
-        const unsigned v = t >= off ? s[t - off] : 0u;
-        __syncthreads();
-        s[t] += v;
-        __syncthreads();
-    }
-    unsigned run = s[t] - sum;
-    for (int k = b0; k < b1; ++k) {
-        const unsigned v = a[k];
-        a[k] = run;
-        run += v;
-    }
-}
-
-__global__ __launch_bounds__(kSortBS) void k_radix_scatter(
+__global__ __launch_bounds__(kSortBS) void k_onesweep(
     const uint64_t *__restrict__ kin, const int *__restrict__ iin, uint64_t *__restrict__ kout,
-    int *__restrict__ iout, int n, int shift, int nblk, const unsigned *__restrict__ offs)
+    int *__restrict__ iout, int n, int shift, const unsigned *__restrict__ ghist_p,
+    unsigned *__restrict__ lookback_p, int *__restrict__ ticket_p,
+    unsigned long long *__restrict__ stats)
 {
     constexpr int kWaves = kSortBS / 64;
     constexpr int kRounds = kSortItems;  // each wave: kRounds rounds of 64 keys
     __shared__ unsigned s_base[kWaves][256];
+    __shared__ unsigned s_scan[256];
+    __shared__ int s_tile;
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int lane = t & 63;
+    if (t == 0) s_tile = atomicAdd(ticket_p, 1);
     for (int k = t; k < kWaves * 256; k += kSortBS) (&s_base[0][0])[k] = 0u;
+    s_scan[t] = ghist_p[t];
     __syncthreads();
+    // exclusive scan of this pass's global digit counts
+    for (int off = 1; off < 256; off <<= 1) {
+        const unsigned v = t >= off ? s_scan[t - off] : 0u;
+        __syncthreads();
+        s_scan[t] += v;
+        __syncthreads();
+    }
+    const unsigned goff = s_scan[t] - ghist_p[t];
+    const int tile = s_tile;
 
-    const int seg = blockIdx.x * kSortTile + w * (kRounds * 64);
+    const int seg = tile * kSortTile + w * (kRounds * 64);
     uint64_t key[kRounds];
     int idv[kRounds];
 #pragma unroll
@@ -291,9 +295,38 @@ __global__ __launch_bounds__(kSortBS) void k_radix_scatter(
         }
     }
     __syncthreads();
+    // thread t owns digit t
+    unsigned cnt = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) cnt += s_base[ww][t];
+    unsigned *mine = &lookback_p[tile * 256 + t];
+    __hip_atomic_store(mine, (tile == 0 ? kFlagInc : kFlagAgg) | cnt, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    unsigned excl = 0;
+    if (tile > 0) {
+        int j = tile - 1;
+        int spins = 0;
+        while (j >= 0) {
+            const unsigned v = __hip_atomic_load(&lookback_p[j * 256 + t], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned f = v & ~kCountMask;
+            if (f == 0u) {
+                if (++spins > kSpinLimit) {
+                    atomicAdd(&stats[3], 1ull);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += v & kCountMask;
+            if (f == kFlagInc) break;
+            --j;
+        }
+        __hip_atomic_store(mine, kFlagInc | (excl + cnt), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
     {
-        // Thread t owns digit t: wave-ordered bases from the global offset.
-        unsigned run = offs[t * nblk + blockIdx.x];
+        unsigned run = goff + excl;
 #pragma unroll
         for (int ww = 0; ww < kWaves; ++ww) {
             const unsigned c = s_base[ww][t];
@@ -317,8 +350,9 @@ __global__ __launch_bounds__(kSortBS) void k_radix_scatter(
         const unsigned rank = __popcll(peers & lt_mask);
         const unsigned base = s_base[w][d];
         if (valid) {
-            kout[base + rank] = key[r];
-            iout[base + rank] = idv[r];
+            const unsigned dst = min(base + rank, static_cast<unsigned>(n - 1));
+            kout[dst] = key[r];
+            iout[dst] = idv[r];
             if (rank == 0) s_base[w][d] = base + static_cast<unsigned>(__popcll(peers));
         }
     }
@@ -938,7 +972,9 @@ struct rg_bh {
 
     uint64_t *keys[2] = {nullptr, nullptr};
     int *idx[2] = {nullptr, nullptr};
-    unsigned *hist = nullptr;
+    unsigned *ghist = nullptr;    // kMaxPasses x 256 global digit counts
+    unsigned *lookback = nullptr; // passes x nblk x 256 look-back words
+    int *tickets = nullptr;
     int *left = nullptr, *right = nullptr, *parent = nullptr;
     int2 *range = nullptr;
     int *left_at_split = nullptr, *arrive = nullptr;
@@ -1046,7 +1082,9 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->keys[1], 8 * N);
     ALLOC(h->idx[0], 4 * N);
     ALLOC(h->idx[1], 4 * N);
-    ALLOC(h->hist, 4 * 256 * static_cast<size_t>(h->nblk));
+    ALLOC(h->ghist, 4 * 256 * kMaxPasses);
+    ALLOC(h->tickets, 4 * kMaxPasses);
+    ALLOC(h->lookback, 4 * 256 * static_cast<size_t>(h->nblk) * h->passes);
     ALLOC(h->left, 4 * (N - 1));
     ALLOC(h->right, 4 * (N - 1));
     ALLOC(h->range, 8 * (N - 1));
@@ -1095,20 +1133,19 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
     double *box = h->box;
     int rc = bh_mark(h, 0, st);
     if (rc != RG_OK) return rc;
-    rc = launch_bbox_morton(pos, n, h->bits, part, box, h->stats, h->keys[0], st);
+    const int lb_words = 256 * h->nblk * h->passes;
+    rc = launch_bbox_morton(pos, n, h->bits, part, box, h->stats, h->keys[0], h->passes,
+                            h->ghist, h->tickets, h->lookback, lb_words, st);
     if (rc != RG_OK) return rc;
 
-    // Stable LSD radix sort, 8-bit digits.
+    // Stable LSD radix sort, 8-bit digits, one kernel per pass.
     int cur = 0;
     for (int p = 0; p < h->passes; ++p) {
-        const int shift = 8 * p;
-        k_radix_hist<<<h->nblk, kSortBS, 0, st>>>(h->keys[cur], n, shift, h->nblk, h->hist);
-        RG_CHECK_LAUNCH();
-        k_scan_exclusive<<<1, kScanBS, 0, st>>>(h->hist, 256 * h->nblk);
-        RG_CHECK_LAUNCH();
-        k_radix_scatter<<<h->nblk, kSortBS, 0, st>>>(h->keys[cur], p == 0 ? nullptr : h->idx[cur],
-                                                     h->keys[cur ^ 1], h->idx[cur ^ 1], n, shift,
-                                                     h->nblk, h->hist);
+        k_onesweep<<<h->nblk, kSortBS, 0, st>>>(h->keys[cur], p == 0 ? nullptr : h->idx[cur],
+                                                h->keys[cur ^ 1], h->idx[cur ^ 1], n, 8 * p,
+                                                h->ghist + 256 * p,
+                                                h->lookback + static_cast<size_t>(256) * h->nblk * p,
+                                                h->tickets + p, h->stats);
         RG_CHECK_LAUNCH();
         cur ^= 1;
     }
@@ -1296,5 +1333,6 @@ extern "C" int rg_morton_codes(const double *pos, int64_t n, int bits, uint64_t 
     const int st = rg::device_scratch(8 * bbox_part_doubles(ni), &scratch);
     if (st != RG_OK) return st;
     double *box = static_cast<double *>(scratch);
-    return launch_bbox_morton(pos, ni, bits, box + 8, box, nullptr, keys, rg::as_stream(stream));
+    return launch_bbox_morton(pos, ni, bits, box + 8, box, nullptr, keys, 0, nullptr, nullptr,
+                              nullptr, 0, rg::as_stream(stream));
 }
