@@ -13,11 +13,11 @@
 //   4. k_build     Karras build with the reference's exact delta (incl. the
 //                  index tie-break) using hardware clz (:50-121)
 //   5. k_agg_*     leaf init + deterministic bottom-up aggregation: the
-//                  second child to finish combines both children (acq_rel
-//                  arrival counters; LDS and workgroup scope inside each
-//                  treelet of <= 256 leaves, one workgroup for the top tree
-//                  above them), so every node is computed exactly once and
-//                  bitwise reproducibly (:124-177)
+//                  second child to finish combines both children, so every
+//                  node is computed exactly once and bitwise reproducibly
+//                  (:124-177).  Three levels, each synchronised inside one
+//                  workgroup: treelets of <= 256 leaves in LDS, mid-treelets
+//                  of <= 65536 leaves, and the top tree above them.
 //   6. k_walk*     stackless walk in right-first DFS order: the node
 //                  sequence the reference's stack walk pops (:180-217).
 //                  FP64 follows right-child/escape links over node-id
@@ -377,9 +377,11 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
                         int *__restrict__ right, int *__restrict__ parent,
                         int *__restrict__ up, int2 *__restrict__ range,
                         int *__restrict__ left_at_split, int *__restrict__ arrive,
-                        int *__restrict__ tl_count)
+                        int *__restrict__ counts, int *__restrict__ tl_of)
 {
     const int64_t nn = n;
+    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += gridDim.x * blockDim.x)
+        tl_of[li] = -1;
     for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < n - 1;
          ii += gridDim.x * blockDim.x) {
         const int64_t i = ii;
@@ -418,7 +420,8 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
         if (i == 0) {
             parent[nn] = -1;
             up[nn] = -1;
-            *tl_count = 0;
+            counts[0] = 0;  // treelets
+            counts[1] = 0;  // mid-treelets
         }
     }
 }
@@ -449,9 +452,6 @@ __global__ void k_preorder(const int *__restrict__ up, const int2 *__restrict__ 
 }
 
 // --------------------------------------------------------- aggregation
-struct NodeArrays {
-    double *com, *nmass, *nmin, *nmax, *nmom;
-};
 
 // Node aggregate used when combining children: one 80-byte record.
 struct Agg {
@@ -476,7 +476,7 @@ struct WalkOut {
 };
 
 template <int MODE>
-__device__ __forceinline__ void write_node(const NodeArrays &na, Agg *agg, const WalkOut &wo,
+__device__ __forceinline__ void write_node(Agg *agg, const WalkOut &wo,
                                            int node, int n, double m, const double mom[3],
                                            const double mn[3], const double mx[3],
                                            const double c[3], int lo, int hi, int right_child,
@@ -491,14 +491,6 @@ __device__ __forceinline__ void write_node(const NodeArrays &na, Agg *agg, const
         g.mx[a] = mx[a];
     }
     agg[node] = g;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        na.nmom[3 * node + a] = mom[a];
-        na.nmin[3 * node + a] = mn[a];
-        na.nmax[3 * node + a] = mx[a];
-        na.com[3 * node + a] = c[a];
-    }
-    na.nmass[node] = m;
     double size = mx[0] - mn[0];
     size = mx[1] - mn[1] > size ? mx[1] - mn[1] : size;
     size = mx[2] - mn[2] > size ? mx[2] - mn[2] : size;
@@ -521,22 +513,29 @@ __device__ __forceinline__ void write_node(const NodeArrays &na, Agg *agg, const
     }
 }
 
-constexpr int kTreelet = 256;  // max leaves of a treelet (one workgroup)
+constexpr int kTreelet = 256;    // max leaves of a treelet (one workgroup, LDS)
+constexpr int kMidTree = 65536;  // max leaves of a mid-treelet (one workgroup, global)
 
 // Treelets: the maximal subtrees with at most kTreelet leaves (a node is a
 // treelet root when its leaf count is <= kTreelet and its parent's is
-// larger).  They partition the leaves; the internal nodes above them form
-// the top tree.
+// larger); mid-treelets likewise with kMidTree.  Both partition the leaves.
+// tl_of[first leaf of a treelet] = its root (other leaves keep -1).
 __global__ void k_classify(const int *__restrict__ parent, const int2 *__restrict__ range, int n,
-                           int *__restrict__ treelets, int *__restrict__ tl_count)
+                           int *__restrict__ treelets, int *__restrict__ mids,
+                           int *__restrict__ counts, int *__restrict__ tl_of)
 {
     const int m = 2 * n - 1;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m; v += gridDim.x * blockDim.x) {
-        const int size = v < n ? 1 : range[v - n].y - range[v - n].x + 1;
-        if (size > kTreelet) continue;
+        const int lo = v < n ? v : range[v - n].x;
+        const int size = v < n ? 1 : range[v - n].y - lo + 1;
+        if (size > kMidTree) continue;
         const int p = parent[v];
         const int psize = p < 0 ? INT_MAX : range[p - n].y - range[p - n].x + 1;
-        if (psize > kTreelet) treelets[atomicAdd(tl_count, 1)] = v;
+        if (size <= kTreelet && psize > kTreelet) {
+            treelets[atomicAdd(&counts[0], 1)] = v;
+            tl_of[lo] = v;
+        }
+        if (psize > kMidTree) mids[atomicAdd(&counts[1], 1)] = v;
     }
 }
 
@@ -550,7 +549,7 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     const double *__restrict__ mass, int n, const int *__restrict__ left,
     const int *__restrict__ right, const int *__restrict__ parent,
     const int2 *__restrict__ range, const int *__restrict__ left_at_split,
-    const int *__restrict__ treelets, const int *__restrict__ tl_count, NodeArrays na, Agg *agg,
+    const int *__restrict__ treelets, const int *__restrict__ counts, Agg *agg,
     WalkOut wo)
 {
     constexpr int C = kTreelet;
@@ -560,7 +559,7 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     __shared__ double s_mx[3][2 * C];
     __shared__ int s_arrive[C];
     const int tid = threadIdx.x;
-    const int cnt = *tl_count;
+    const int cnt = counts[0];
     for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
         const int v = treelets[e];
         const int lo = v < n ? v : range[v - n].x;
@@ -582,14 +581,19 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
                 s_mx[a][tid] = x[a];
             }
             s_m[tid] = m;
-            write_node<MODE>(na, agg, wo, li, n, m, mom, x, x, x, li, li, -1,
+            write_node<MODE>(agg, wo, li, n, m, mom, x, x, x, li, li, -1,
                              li == 0 ? -1 : left_at_split[li - 1]);
             int node = li == v ? -1 : parent[li];
             while (node >= 0) {
                 const int ii = node - n;
-                if (__hip_atomic_fetch_add(&s_arrive[ii - lo], 1, __ATOMIC_ACQ_REL,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-                    break;  // the sibling subtree is not finished; its thread continues
+                // Only the LDS node data must be ordered with the arrival
+                // counter (the global outputs are read by later kernels), so
+                // the fences are LDS-only and do not wait for global stores.
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                const int old = __hip_atomic_fetch_add(&s_arrive[ii - lo], 1, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                if (old == 0) break;  // the sibling subtree is not finished; its thread continues
                 const int L = left[ii], R = right[ii];
                 const int2 rg = range[ii];
                 const int sL = L < n ? L - lo : C + (L - n - lo);
@@ -612,7 +616,7 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
                     s_mn[a][sn] = mn[a];
                     s_mx[a][sn] = mx[a];
                 }
-                write_node<MODE>(na, agg, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
+                write_node<MODE>(agg, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
                                  rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
                 node = node == v ? -1 : parent[node];
             }
@@ -621,11 +625,93 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     }
 }
 
-// Pass 2: one workgroup finishes the top tree.  Each treelet root is an
-// arrival at its parent; the second arriver at a node combines the two
-// children's aggregates (treelet roots from pass 1, visible across the
-// kernel boundary; top nodes from this workgroup, published by the
-// workgroup-scope acq_rel arrival counter).
+// A node's topology, loaded before its arrival counter so the loads overlap
+// the atomic's round trip.
+struct Topo {
+    int L, R, up;
+    int2 rg;
+};
+
+__device__ __forceinline__ Topo load_topo(int node, int n, const int *__restrict__ left,
+                                          const int *__restrict__ right,
+                                          const int *__restrict__ parent,
+                                          const int2 *__restrict__ range)
+{
+    const int ii = node - n;
+    return Topo{left[ii], right[ii], parent[node], range[ii]};
+}
+
+// Combine a node from its children's aggregates (global) and write it.
+template <int MODE>
+__device__ __forceinline__ void combine_global(int node, int n, const Topo &tp,
+                                               const int *__restrict__ left_at_split,
+                                               Agg *agg, const WalkOut &wo)
+{
+    const Agg aL = agg[tp.L];
+    const Agg aR = agg[tp.R];
+    const double mm = aL.m + aR.m;
+    double mom[3], mn[3], mx[3], c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        mom[a] = aL.mom[a] + aR.mom[a];
+        mn[a] = fmin(aL.mn[a], aR.mn[a]);
+        mx[a] = fmax(aL.mx[a], aR.mx[a]);
+        c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
+    }
+    write_node<MODE>(agg, wo, node, n, mm, mom, mn, mx, c, tp.rg.x, tp.rg.y, tp.R,
+                     tp.rg.x == 0 ? -1 : left_at_split[tp.rg.x - 1]);
+}
+
+// Pass 2: one workgroup per mid-treelet.  Its threads find the treelet
+// roots inside it (tl_of on the first leaves) and climb from each; both
+// children of every node they meet lie inside the mid-treelet, so the
+// arrival counters (global) and the node data are synchronised at
+// workgroup scope (acq_rel).  The climb stops at the mid-treelet root.
+constexpr int kMidBS = 1024;
+
+template <int MODE>
+__global__ __launch_bounds__(kMidBS) void k_agg_mid(
+    int n, const int *__restrict__ left, const int *__restrict__ right,
+    const int *__restrict__ parent, const int2 *__restrict__ range,
+    const int *__restrict__ left_at_split, int *__restrict__ arrive,
+    const int *__restrict__ mids, const int *__restrict__ counts,
+    const int *__restrict__ tl_of, int *__restrict__ entries, Agg *agg, WalkOut wo)
+{
+    __shared__ int s_cnt;
+    const int cnt = counts[1];
+    for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
+        const int M = mids[e];  // uniform
+        if (M < n) continue;    // a single leaf
+        const int2 mr = range[M - n];
+        // Collect the treelet roots inside M (the scan's loads are independent)
+        // into entries[mr.x ...] -- M's own leaf range, so no overflow and no
+        // overlap with another mid-treelet -- then climb from all of them.
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+#pragma unroll 4
+        for (int li = mr.x + static_cast<int>(threadIdx.x); li <= mr.y; li += kMidBS) {
+            const int r = tl_of[li];
+            if (r >= 0 && r != M) entries[mr.x + atomicAdd(&s_cnt, 1)] = r;
+        }
+        __syncthreads();  // workgroup-scope visibility of entries[] (global) and s_cnt
+        const int ne = s_cnt;
+        for (int k = threadIdx.x; k < ne; k += kMidBS) {
+            int node = parent[entries[mr.x + k]];
+            while (true) {
+                const Topo tp = load_topo(node, n, left, right, parent, range);
+                if (__hip_atomic_fetch_add(&arrive[node - n], 1, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                    break;
+                combine_global<MODE>(node, n, tp, left_at_split, agg, wo);
+                if (node == M) break;
+                node = tp.up;
+            }
+        }
+        __syncthreads();  // s_cnt is reused by the next mid-treelet
+    }
+}
+
+// Pass 3: one workgroup climbs from the mid-treelet roots to the root.
 constexpr int kTopBS = 1024;
 
 template <int MODE>
@@ -633,34 +719,41 @@ __global__ __launch_bounds__(kTopBS) void k_agg_top(
     int n, const int *__restrict__ left, const int *__restrict__ right,
     const int *__restrict__ parent, const int2 *__restrict__ range,
     const int *__restrict__ left_at_split, int *__restrict__ arrive,
-    const int *__restrict__ treelets, const int *__restrict__ tl_count, NodeArrays na, Agg *agg,
+    const int *__restrict__ mids, const int *__restrict__ counts, Agg *agg,
     WalkOut wo)
 {
-    const int cnt = *tl_count;
+    const int cnt = counts[1];
     for (int e = threadIdx.x; e < cnt; e += kTopBS) {
-        int node = parent[treelets[e]];
+        int node = parent[mids[e]];
         while (node >= 0) {
-            const int ii = node - n;
-            if (__hip_atomic_fetch_add(&arrive[ii], 1, __ATOMIC_ACQ_REL,
+            const Topo tp = load_topo(node, n, left, right, parent, range);
+            if (__hip_atomic_fetch_add(&arrive[node - n], 1, __ATOMIC_ACQ_REL,
                                        __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
                 break;
-            const int L = left[ii], R = right[ii];
-            const int2 rg = range[ii];
-            const Agg aL = agg[L];
-            const Agg aR = agg[R];
-            const double mm = aL.m + aR.m;
-            double mom[3], mn[3], mx[3], c[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                mom[a] = aL.mom[a] + aR.mom[a];
-                mn[a] = fmin(aL.mn[a], aR.mn[a]);
-                mx[a] = fmax(aL.mx[a], aR.mx[a]);
-                c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
-            }
-            write_node<MODE>(na, agg, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
-                             rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
-            node = parent[node];
+            combine_global<MODE>(node, n, tp, left_at_split, agg, wo);
+            node = tp.up;
         }
+    }
+}
+
+// Reference-layout node arrays (barnes_hut.py:232-236) from the aggregates,
+// on demand: com is the leaf position for leaves (exactly its AABB) and
+// mom / mass (0 when massless) for internal nodes -- the expression the
+// walk records use.
+__global__ void k_export_nodes(const Agg *__restrict__ agg, int n, double *__restrict__ com,
+                               double *__restrict__ nmass, double *__restrict__ nmin,
+                               double *__restrict__ nmax)
+{
+    const int m = 2 * n - 1;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m; v += gridDim.x * blockDim.x) {
+        const Agg g = agg[v];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (com) com[3 * v + a] = v < n ? g.mn[a] : (g.m > 0.0 ? g.mom[a] / g.m : 0.0);
+            if (nmin) nmin[3 * v + a] = g.mn[a];
+            if (nmax) nmax[3 * v + a] = g.mx[a];
+        }
+        if (nmass) nmass[v] = g.m;
     }
 }
 
@@ -978,9 +1071,9 @@ struct rg_bh {
     int *left = nullptr, *right = nullptr, *parent = nullptr;
     int2 *range = nullptr;
     int *left_at_split = nullptr, *arrive = nullptr;
-    int *up = nullptr, *pre = nullptr, *treelets = nullptr, *tl_count = nullptr;
+    int *up = nullptr, *pre = nullptr, *treelets = nullptr, *mids = nullptr;
+    int *counts = nullptr, *tl_of = nullptr, *entries = nullptr;
     Agg *agg = nullptr;
-    NodeArrays na{};
     WalkRec64 *rec64 = nullptr;  // FP64 walk records (node-id order)
     float4 *recA = nullptr;      // FP32 walk records (preorder)
     int2 *recB = nullptr;
@@ -1091,14 +1184,12 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->left_at_split, 4 * (N - 1));
     ALLOC(h->arrive, 4 * (N - 1));
     ALLOC(h->parent, 4 * M);
-    ALLOC(h->na.com, 24 * M);
-    ALLOC(h->na.nmass, 8 * M);
-    ALLOC(h->na.nmin, 24 * M);
-    ALLOC(h->na.nmax, 24 * M);
-    ALLOC(h->na.nmom, 24 * M);
     ALLOC(h->up, 4 * M);
     ALLOC(h->treelets, 4 * M);
-    ALLOC(h->tl_count, 16);
+    ALLOC(h->mids, 4 * M);
+    ALLOC(h->counts, 16);
+    ALLOC(h->tl_of, 4 * N);
+    ALLOC(h->entries, 4 * N);
     ALLOC(h->agg, sizeof(Agg) * M);
     if (precision == RG_FP64) {
         ALLOC(h->rec64, sizeof(WalkRec64) * M);
@@ -1157,12 +1248,13 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
 
     const int g1 = min((n + 255) / 256, 16384);
     k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->up, h->range,
-                                h->left_at_split, h->arrive, h->tl_count);
+                                h->left_at_split, h->arrive, h->counts, h->tl_of);
     RG_CHECK_LAUNCH();
     const bool fp64 = h->precision == RG_FP64;
     {
         const int gc = min((2 * n - 1 + 255) / 256, 16384);
-        k_classify<<<gc, 256, 0, st>>>(h->parent, h->range, n, h->treelets, h->tl_count);
+        k_classify<<<gc, 256, 0, st>>>(h->parent, h->range, n, h->treelets, h->mids, h->counts,
+                                       h->tl_of);
         RG_CHECK_LAUNCH();
     }
     if (!fp64) {
@@ -1182,24 +1274,28 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
     wo.inv_theta2 = wo.theta_zero ? 0.0 : 1.0 / theta2;
     rc = bh_mark(h, 2, st);
     if (rc != RG_OK) return rc;
-    const int gt = min(n, 4096);
-    if (fp64) {
-        k_agg_treelet<0><<<gt, kTreelet, 0, st>>>(order, pos, mass, n, h->left, h->right,
-                                                 h->parent, h->range, h->left_at_split,
-                                                 h->treelets, h->tl_count, h->na, h->agg, wo);
-        RG_CHECK_LAUNCH();
-        k_agg_top<0><<<1, kTopBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,
-                                           h->left_at_split, h->arrive, h->treelets,
-                                           h->tl_count, h->na, h->agg, wo);
-    } else {
-        k_agg_treelet<1><<<gt, kTreelet, 0, st>>>(order, pos, mass, n, h->left, h->right,
-                                                 h->parent, h->range, h->left_at_split,
-                                                 h->treelets, h->tl_count, h->na, h->agg, wo);
-        RG_CHECK_LAUNCH();
-        k_agg_top<1><<<1, kTopBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,
-                                           h->left_at_split, h->arrive, h->treelets,
-                                           h->tl_count, h->na, h->agg, wo);
-    }
+    const int gt = min(n / 64 + 64, 4096);
+    const int gm = min(n / 16384 + 8, 1024);
+#define RG_AGG_LAUNCH(MODE)                                                                  \
+    do {                                                                                     \
+        k_agg_treelet<MODE><<<gt, kTreelet, 0, st>>>(order, pos, mass, n, h->left, h->right, \
+                                                    h->parent, h->range, h->left_at_split,   \
+                                                    h->treelets, h->counts, h->agg,   \
+                                                    wo);                                     \
+        RG_CHECK_LAUNCH();                                                                   \
+        k_agg_mid<MODE><<<gm, kMidBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,    \
+                                               h->left_at_split, h->arrive, h->mids,         \
+                                               h->counts, h->tl_of, h->entries, h->agg, wo);      \
+        RG_CHECK_LAUNCH();                                                                   \
+        k_agg_top<MODE><<<1, kTopBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,     \
+                                              h->left_at_split, h->arrive, h->mids,          \
+                                              h->counts, h->agg, wo);                 \
+    } while (0)
+    if (fp64)
+        RG_AGG_LAUNCH(0);
+    else
+        RG_AGG_LAUNCH(1);
+#undef RG_AGG_LAUNCH
     RG_CHECK_LAUNCH();
     rc = bh_mark(h, 3, st);
     if (rc != RG_OK) return rc;
@@ -1283,13 +1379,10 @@ extern "C" int rg_bh_export_nodes(rg_bh *h, double *com, double *nmass, double *
 {
     RG_REQUIRE(h, RG_ERR_ARG, "null handle");
     RG_REQUIRE(h->computed, RG_ERR_ARG, "rg_bh_compute has not run on this handle");
-    hipStream_t st = rg::as_stream(stream);
-    const size_t M = 2 * static_cast<size_t>(h->n) - 1;
-    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
-    if (com) RG_HIP(hipMemcpyAsync(com, h->na.com, 24 * M, k, st));
-    if (nmass) RG_HIP(hipMemcpyAsync(nmass, h->na.nmass, 8 * M, k, st));
-    if (nmin) RG_HIP(hipMemcpyAsync(nmin, h->na.nmin, 24 * M, k, st));
-    if (nmax) RG_HIP(hipMemcpyAsync(nmax, h->na.nmax, 24 * M, k, st));
+    const int n = static_cast<int>(h->n);
+    const int g = min((2 * n - 1 + 255) / 256, 16384);
+    k_export_nodes<<<g, 256, 0, rg::as_stream(stream)>>>(h->agg, n, com, nmass, nmin, nmax);
+    RG_CHECK_LAUNCH();
     return RG_OK;
 }
 
