@@ -55,7 +55,8 @@ const char *rg_last_error(void);
  *                 1, 2 or 4
  *   "walk_block"  threads per block of the BH walk: 64, 128, 256 or 512
  *   "walk_mode"   FP32 BH walk: 0 one lane per particle, 1 two lanes per
- *                 particle, 2 one wave walking together (default)
+ *                 particle, 2 one wave walking together (default), 3 the
+ *                 same advancing two slots per step
  * Process-wide; takes effect at the next launch. */
 int rg_set_param(const char *name, int64_t value);
 

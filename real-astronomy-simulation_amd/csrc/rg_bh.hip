@@ -49,7 +49,7 @@ constexpr int kBoxBS = 256;
 constexpr int kWalkBSDefault = 256;
 
 int g_walk_bs = kWalkBSDefault;
-int g_walk_mode = 2;  // FP32 walk: 0 one lane per particle, 1 two lanes, 2 wave-coherent
+int g_walk_mode = 2;  // FP32 walk: 0 one lane, 1 two lanes, 2 wave-coherent, 3 wave, two slots per step
 
 struct alignas(16) WalkRec64 {
     double x, y, z, m;   // centre of mass, mass
@@ -224,7 +224,8 @@ int launch_bbox_morton(const double *pos, int n, int bits, double *part, double 
     RG_CHECK_LAUNCH();
     k_bbox_final<<<1, kBoxBS, 0, st>>>(part, nb, bits, box, stats, ghist, tickets);
     RG_CHECK_LAUNCH();
-    const int mg = min((n + 255) / 256, 2048);
+    // each block flushes passes x 256 global atomics: keep the grid modest
+    const int mg = min((n + 255) / 256, passes > 0 ? 256 : 2048);
     k_morton<<<mg, 256, 0, st>>>(pos, n, bits, box, keys, passes, ghist, lookback, lb_words);
     RG_CHECK_LAUNCH();
     return RG_OK;
@@ -1048,6 +1049,114 @@ __global__ __launch_bounds__(256) void k_walk32_wave(const float4 *__restrict__ 
     }
 }
 
+// The wave walk advancing two slots per iteration: slots k and k+1 (usually
+// the next two the wave needs, since opening a node leads to k+1) are
+// evaluated together -- their arithmetic is independent, so it overlaps --
+// and applied in slot order, keeping each lane's participation and
+// summation order exactly those of the one-slot walk.
+__global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__ recA,
+                                                      const int2 *__restrict__ recB,
+                                                      const int *__restrict__ pre,
+                                                      const int *__restrict__ order,
+                                                      double *__restrict__ acc, int n,
+                                                      float eps2,
+                                                      unsigned long long *__restrict__ stats)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int t = b * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool active = t < n;
+    const int m = 2 * n - 1;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (active) {
+        const float4 self = recA[pre[t]];
+        px = self.x;
+        py = self.y;
+        pz = self.z;
+    }
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    int visits = 0;
+    int next = active ? 0 : m;
+    int k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+    int k0 = -64;
+    float4 wa = make_float4(0.f, 0.f, 0.f, 0.f);
+    int2 wb = make_int2(0, 0);
+    while (k < m) {
+        if (k + 1 - k0 >= 64) {  // the window must hold slots k and k + 1
+            k0 = k;
+            const int slot = k0 + lane;
+            if (slot < m) {
+                wa = recA[slot];
+                wb = recB[slot];
+            }
+        }
+        const int l = k - k0;
+        const bool has1 = k + 1 < m;  // uniform
+        float c[2][4], crit[2];
+        int skip[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            c[q][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l + q));
+            c[q][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l + q));
+            c[q][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l + q));
+            c[q][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l + q));
+            crit[q] = __int_as_float(__builtin_amdgcn_readlane(wb.x, l + q));
+            skip[q] = __builtin_amdgcn_readlane(wb.y, l + q);
+        }
+        float dx[2], dy[2], dz[2], r2[2], s[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            dx[q] = c[q][0] - px;
+            dy[q] = c[q][1] - py;
+            dz[q] = c[q][2] - pz;
+            r2[q] = fmaf(dz[q], dz[q], fmaf(dy[q], dy[q], fmaf(dx[q], dx[q], eps2)));
+            const float inv = __builtin_amdgcn_rsqf(r2[q]);
+            s[q] = c[q][3] * (inv * inv) * inv;
+        }
+        if (next == k) {
+            ++visits;
+            if (crit[0] < r2[0]) {
+                ax = fmaf(s[0], dx[0], ax);
+                ay = fmaf(s[0], dy[0], ay);
+                az = fmaf(s[0], dz[0], az);
+                next = max(skip[0], k + 1);
+            } else {
+                next = k + 1;
+            }
+        }
+        if (has1 && next == k + 1) {
+            ++visits;
+            if (crit[1] < r2[1]) {
+                ax = fmaf(s[1], dx[1], ax);
+                ay = fmaf(s[1], dy[1], ay);
+                az = fmaf(s[1], dz[1], az);
+                next = max(skip[1], k + 2);
+            } else {
+                next = k + 2;
+            }
+        }
+        // every lane now wants a slot >= kk; kk itself whenever a lane
+        // opened (or stepped onto) slot k + 1
+        const int kk = has1 ? k + 2 : k + 1;
+        if (__ballot(next == kk))
+            k = kk;
+        else
+            k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+    }
+    if (active) {
+        const int p = order[t];
+        acc[3 * p] = static_cast<double>(ax);
+        acc[3 * p + 1] = static_cast<double>(ay);
+        acc[3 * p + 2] = static_cast<double>(az);
+    }
+    const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
+    const int mx = wave_max_i32(visits);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], tot);
+        atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+    }
+}
+
 }  // namespace
 
 // ================================================================ handle
@@ -1117,7 +1226,7 @@ int bh_mark(rg_bh *h, int phase, hipStream_t st)
 
 int rg::rg_bh_set_walk_mode_impl(int64_t v)
 {
-    RG_REQUIRE(v >= 0 && v <= 2, RG_ERR_ARG, "walk_mode must be 0, 1 or 2");
+    RG_REQUIRE(v >= 0 && v <= 3, RG_ERR_ARG, "walk_mode must be 0, 1, 2 or 3");
     g_walk_mode = static_cast<int>(v);
     return RG_OK;
 }
@@ -1306,7 +1415,11 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
     if (fp64) {
         k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, acc, n, h->G, theta2, eps2, h->stats);
     } else {
-        if (g_walk_mode == 2) {
+        if (g_walk_mode == 3) {
+            k_walk32_wave2<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
+                                                            acc, n, static_cast<float>(eps2),
+                                                            h->stats);
+        } else if (g_walk_mode == 2) {
             k_walk32_wave<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
                                                            acc, n, static_cast<float>(eps2),
                                                            h->stats);

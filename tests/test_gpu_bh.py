@@ -182,16 +182,33 @@ def test_fp32_walk_modes_agree(name):
     from rgsim import _lib
     pos, mass = CASES[name]
     res = {}
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3):
         _lib.set_param("walk_mode", mode)
         try:
             bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
             res[mode] = (acc, bh.stats())
         finally:
             _lib.set_param("walk_mode", 2)
-    (a0, s0), (a1, s1), (a2, s2) = res[0], res[1], res[2]
-    for s in (s1, s2):
+    (a0, s0), (a1, s1), (a2, s2), (a3, s3) = res[0], res[1], res[2], res[3]
+    for s in (s1, s2, s3):
         assert s0["visits_total"] == s["visits_total"] and s0["visits_max"] == s["visits_max"]
     scale = np.linalg.norm(a0, axis=1).max()
     assert np.abs(a0 - a1).max() <= 1e-5 * scale
-    assert np.array_equal(a0, a2)
+    assert np.array_equal(a0, a2) and np.array_equal(a0, a3)
+
+
+def test_bh_1m_keys_order_topology_bitexact(orc):
+    """The N=1,000,000 treecode configuration (disk:bulge 5:1, seed 1):
+    Morton keys, stable order and the full Karras topology equal the oracle
+    restatement bit for bit."""
+    from rgsim.ic import make_disk_bulge
+    print("bh topology N=1M", flush=True)
+    s = make_disk_bulge(n_disk=833334, n_bulge=166666, seed=1)
+    bh, _ = run_bh(s.pos, s.mass, theta=0.5, eps=0.05, G=43009.1725, precision="fp32")
+    keys, order, left, right, parent = (t.cpu().numpy() for t in bh.export_tree())
+    t = orc.bh_tree(s.pos, s.mass)
+    assert np.array_equal(order, t["order"])
+    assert np.array_equal(keys.astype(np.uint64), t["keys"][t["order"]])
+    assert np.array_equal(left, t["left"])
+    assert np.array_equal(right, t["right"])
+    assert np.array_equal(parent, t["parent"])
