@@ -83,6 +83,11 @@ def bench_bh(steps, warmup):
     visits = st_last["visits_total"]
     walk_bytes = visits * WALK_RECORD_BYTES + n * WALK_PARTICLE_BYTES
     achieved = walk_bytes / (walk_ms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "k_walk32_wave_diskbulge120k_pmc.json")
+    if os.path.exists(pmc):  # rocprofv3 FETCH_SIZE / WRITE_SIZE of this kernel, copy-scaled
+        with open(pmc) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     return {
         "n": n, "wall_s": wall, "ms_per_step": 1e3 * wall / steps,
         "mps": n * steps / wall / 1e6,
@@ -94,8 +99,8 @@ def bench_bh(steps, warmup):
                      "aggregate": tm["aggregate_ms"] / max(tm["count"], 1),
                      "walk": walk_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_walk32", "algorithmic_bytes_per_launch": walk_bytes,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_walk32_wave", "algorithmic_bytes_per_launch": walk_bytes,
                      "avg_launch_ms": walk_ms},
         "energy": None,
     }
@@ -357,6 +362,8 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the BH drop-in workload (no direct/adjoint/batch/1M sections)")
+    ap.add_argument("--sections", default="bh,direct,adjoint,batch,bh1m",
+                    help="comma-separated subset of bh,direct,adjoint,batch,bh1m")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -372,20 +379,33 @@ def main():
         copy_calibration()
         return
 
-    log("bench: BH disk+bulge N=120000 ...")
-    bh = bench_bh(args.steps, args.warmup)
-    log(f"bench: BH {bh['mps']:.2f} Mparticle-steps/s, {bh['ms_per_step']:.3f} ms/step")
-    direct = adjoint = batch = bh1m = None
-    if not args.no_direct and not args.no_extra:
+    sections = set(args.sections.split(","))
+    if args.no_extra:
+        sections &= {"bh"}
+    if args.no_direct:
+        sections.discard("direct")
+    bh = direct = adjoint = batch = bh1m = None
+    if "bh" in sections:
+        log("bench: BH disk+bulge N=120000 ...")
+        bh = bench_bh(args.steps, args.warmup)
+        log(f"bench: BH {bh['mps']:.2f} Mparticle-steps/s, {bh['ms_per_step']:.3f} ms/step")
+    if "direct" in sections:
         log("bench: direct Plummer N=262144 ...")
         direct = bench_direct()
-    if not args.no_extra:
+    if "adjoint" in sections:
         log("bench: adjoint Plummer N=16384 ...")
         adjoint = bench_adjoint()
+    if "batch" in sections:
         log("bench: batch 512 systems ...")
         batch = bench_batch()
+    if "bh1m" in sections:
         log("bench: BH N=1M ...")
         bh1m = bench_bh_1m()
+    if bh is None:
+        print(json.dumps({"sections": sorted(sections), "direct_plummer_262144": direct,
+                          "adjoint_plummer_16384": adjoint, "batch_512_systems": batch,
+                          "bh_disk_bulge_1M": bh1m}))
+        return
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline ...")
