@@ -34,6 +34,9 @@ FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_INTERACTION = 20
 WALK_RECORD_BYTES = 32          # FP32 node record fetched per visit
+# FP32 walk variant (rg_bh_stats walk_mode) -> the kernel it launches
+WALK_KERNELS = {0: "k_walk32", 1: "k_walk32_pair", 2: "k_walk32_wave", 3: "k_walk32_wave2",
+                4: "k_walk32_wave_split", -1: "k_walk64"}
 WALK_PARTICLE_BYTES = 16 + 4 + 24  # own leaf record + order index + FP64 acceleration
 
 
@@ -83,11 +86,14 @@ def bench_bh(steps, warmup):
     visits = st_last["visits_total"]
     walk_bytes = visits * WALK_RECORD_BYTES + n * WALK_PARTICLE_BYTES
     achieved = walk_bytes / (walk_ms * 1e-3) / 1e9
+    kernel = WALK_KERNELS[st_last["walk_mode"]]
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "k_walk32_wave_diskbulge120k_pmc.json")
+    pmc = os.path.join(ROOT, "profiles", f"{kernel}_diskbulge120k_pmc.json")
     if os.path.exists(pmc):  # rocprofv3 FETCH_SIZE / WRITE_SIZE of this kernel, copy-scaled
         with open(pmc) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            rec = json.load(f)
+        if rec.get("kernel") == kernel and rec.get("n") == n:
+            traffic = rec.get("hbm_bytes_per_launch")
     return {
         "n": n, "wall_s": wall, "ms_per_step": 1e3 * wall / steps,
         "mps": n * steps / wall / 1e6,
@@ -100,7 +106,7 @@ def bench_bh(steps, warmup):
                      "walk": walk_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_walk32_wave", "algorithmic_bytes_per_launch": walk_bytes,
+                     "kernel": kernel, "algorithmic_bytes_per_launch": walk_bytes,
                      "avg_launch_ms": walk_ms},
         "energy": None,
     }

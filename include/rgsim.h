@@ -54,9 +54,11 @@ const char *rg_last_error(void);
  *   "direct_ipt"  i-particles per lane in the direct kernels: 0 (heuristic),
  *                 1, 2 or 4
  *   "walk_block"  threads per block of the BH walk: 64, 128, 256 or 512
- *   "walk_mode"   FP32 BH walk: 0 one lane per particle, 1 two lanes per
- *                 particle, 2 one wave walking together (default), 3 the
- *                 same advancing two slots per step
+ *   "walk_mode"   FP32 BH walk: -1 auto (default: 4 below 400k particles,
+ *                 else 2), 0 one lane per particle, 1 two lanes per
+ *                 particle, 2 one wave walking together, 3 the same
+ *                 advancing two slots per step, 4 the wave walk split over
+ *                 two waves per particle group
  * Process-wide; takes effect at the next launch. */
 int rg_set_param(const char *name, int64_t value);
 
@@ -142,6 +144,7 @@ typedef struct {
     int64_t visits_total;   /* nodes examined by the walk, all particles */
     int64_t visits_max;     /* max nodes examined by one particle */
     int64_t nonfinite;      /* non-finite input coordinates seen */
+    int64_t walk_mode;      /* FP32 walk variant used (0..4, see "walk_mode"); -1 FP64 */
 } rg_bh_stats_t;
 
 /* Statistics of the last compute.  Synchronises that compute's stream. */

@@ -49,7 +49,9 @@ constexpr int kBoxBS = 256;
 constexpr int kWalkBSDefault = 256;
 
 int g_walk_bs = kWalkBSDefault;
-int g_walk_mode = 2;  // FP32 walk: 0 one lane, 1 two lanes, 2 wave-coherent, 3 wave, two slots per step
+// FP32 walk: -1 auto, 0 one lane, 1 two lanes, 2 wave-coherent, 3 wave with two
+// slots per step, 4 wave-coherent split over two waves per particle group
+int g_walk_mode = -1;
 
 struct alignas(16) WalkRec64 {
     double x, y, z, m;   // centre of mass, mass
@@ -1157,6 +1159,122 @@ __global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__
     }
 }
 
+// The wave walk with each group of 64 particles split over two waves of one
+// workgroup.  S = the preorder slot of the group's middle particle's own
+// leaf; wave 0 owns the slots [0, S), wave 1 owns [S, 2n-1).  A node is
+// tested, counted and (if accepted) added only by its owning wave.  Wave 1
+// also tests the ancestors of slot S (slot < S, subtree reaching past S):
+// accepting one means wave 0 adds its monopole, so wave 1 skips the
+// subtree; subtrees ending at or before S are stepped over untested.  The
+// visited set per particle is the one-wave walk's; the two FP32 partial
+// sums are added through LDS.  Twice the waves, each half the walk.
+__global__ __launch_bounds__(256) void k_walk32_wave_split(
+    const float4 *__restrict__ recA, const int2 *__restrict__ recB, const int *__restrict__ pre,
+    const int *__restrict__ order, double *__restrict__ acc, int n, float eps2,
+    unsigned long long *__restrict__ stats)
+{
+    __shared__ float s_part[2][64][3];  // wave 1's partial sums, per group
+    __shared__ int s_vis[2][64];
+    const int wave = threadIdx.x >> 6;  // 0..3
+    const int grp = wave >> 1;          // group within the block
+    const int half = wave & 1;
+    const int lane = threadIdx.x & 63;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int t = (b * 2 + grp) * 64 + lane;
+    const bool active = t < n;
+    const int m = 2 * n - 1;
+    int ps = 0;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (active) {
+        ps = pre[t];
+        const float4 self = recA[ps];
+        px = self.x;
+        py = self.y;
+        pz = self.z;
+    }
+    // split slot: the middle particle's own leaf (lane 32, or the last active)
+    const int nact = max(0, min(64, n - (t - lane)));
+    const int S = nact > 0 ? __builtin_amdgcn_readlane(ps, min(32, nact - 1)) : m;
+    const int lo = half ? S : 0;
+    const int hi = half ? m : S;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    int visits = 0;
+    int next = (active && lo < hi) ? 0 : m;
+    int k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+    int k0 = -64;
+    float4 wa = make_float4(0.f, 0.f, 0.f, 0.f);
+    int2 wb = make_int2(0, 0);
+    while (k < hi) {
+        if (k - k0 >= 64) {
+            k0 = k;
+            const int slot = k0 + lane;
+            if (slot < m) {
+                wa = recA[slot];
+                wb = recB[slot];
+            }
+        }
+        const int l = k - k0;
+        const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
+        const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
+        const float cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
+        const float cw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
+        const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+        const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
+        if (next == k) {
+            if (skip <= lo) {
+                next = skip;  // subtree entirely before this wave's range (uniform per node)
+            } else {
+                const bool own = k >= lo;
+                const float dx = cx - px;
+                const float dy = cy - py;
+                const float dz = cz - pz;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+                if (crit < r2) {
+                    if (own) {
+                        ++visits;
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        const float s = cw * (inv * inv) * inv;
+                        ax = fmaf(s, dx, ax);
+                        ay = fmaf(s, dy, ay);
+                        az = fmaf(s, dz, az);
+                    }
+                    next = skip;
+                } else {
+                    visits += own ? 1 : 0;
+                    next = k + 1;
+                }
+            }
+        }
+        if (__ballot(next == k + 1))
+            k = k + 1;
+        else
+            k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+    }
+    if (half == 1) {
+        s_part[grp][lane][0] = ax;
+        s_part[grp][lane][1] = ay;
+        s_part[grp][lane][2] = az;
+        s_vis[grp][lane] = visits;
+    }
+    __syncthreads();
+    int pv = 0;
+    if (half == 0) {
+        pv = visits + s_vis[grp][lane];
+        if (active) {
+            const int p = order[t];
+            acc[3 * p] = static_cast<double>(ax + s_part[grp][lane][0]);
+            acc[3 * p + 1] = static_cast<double>(ay + s_part[grp][lane][1]);
+            acc[3 * p + 2] = static_cast<double>(az + s_part[grp][lane][2]);
+        }
+    }
+    const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
+    const int mx = wave_max_i32(pv);
+    if (lane == 0) {
+        atomicAdd(&stats[0], tot);
+        if (half == 0) atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+    }
+}
+
 }  // namespace
 
 // ================================================================ handle
@@ -1171,6 +1289,7 @@ struct rg_bh {
     int final_buf = 0;      // ping-pong buffer holding the sorted result
     hipStream_t last_stream = nullptr;
     bool computed = false;
+    int walk_used = -1;  // walk variant of the last compute (-1 FP64)
 
     uint64_t *keys[2] = {nullptr, nullptr};
     int *idx[2] = {nullptr, nullptr};
@@ -1226,7 +1345,7 @@ int bh_mark(rg_bh *h, int phase, hipStream_t st)
 
 int rg::rg_bh_set_walk_mode_impl(int64_t v)
 {
-    RG_REQUIRE(v >= 0 && v <= 3, RG_ERR_ARG, "walk_mode must be 0, 1, 2 or 3");
+    RG_REQUIRE(v >= -1 && v <= 4, RG_ERR_ARG, "walk_mode must be -1 (auto) or 0..4");
     g_walk_mode = static_cast<int>(v);
     return RG_OK;
 }
@@ -1412,18 +1531,28 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
     const int wb = g_walk_bs;
     const int wg = (n + wb - 1) / wb;
     const double eps2 = h->eps * h->eps;
+    h->walk_used = -1;
     if (fp64) {
         k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, acc, n, h->G, theta2, eps2, h->stats);
     } else {
-        if (g_walk_mode == 3) {
+        // Below ~400k particles the walk has too few waves to hide its step
+        // latency, and splitting each group over two waves pays; above, the
+        // split's extra ancestor tests cost more than it hides.
+        const int mode = g_walk_mode >= 0 ? g_walk_mode : (n < 400000 ? 4 : 2);
+        h->walk_used = mode;
+        if (mode == 4) {
+            k_walk32_wave_split<<<(n + 127) / 128, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
+                                                                 acc, n, static_cast<float>(eps2),
+                                                                 h->stats);
+        } else if (mode == 3) {
             k_walk32_wave2<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
                                                             acc, n, static_cast<float>(eps2),
                                                             h->stats);
-        } else if (g_walk_mode == 2) {
+        } else if (mode == 2) {
             k_walk32_wave<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
                                                            acc, n, static_cast<float>(eps2),
                                                            h->stats);
-        } else if (g_walk_mode == 1) {
+        } else if (mode == 1) {
             const int wg2 = (2 * n + wb - 1) / wb;
             k_walk32_pair<<<wg2, wb, 0, st>>>(h->recA, h->recB, h->pre, order, acc, n,
                                               static_cast<float>(eps2), h->stats);
@@ -1510,6 +1639,7 @@ extern "C" int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out)
     out->visits_total = static_cast<int64_t>(s[0]);
     out->visits_max = static_cast<int64_t>(s[1]);
     out->nonfinite = static_cast<int64_t>(s[2]);
+    out->walk_mode = h->walk_used;
     RG_REQUIRE(s[3] == 0, RG_ERR_CAPACITY,
                "walk exceeded 2n node visits for %llu particles (corrupt tree)", s[3]);
     return RG_OK;

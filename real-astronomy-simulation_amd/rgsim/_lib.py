@@ -30,7 +30,8 @@ _i = ctypes.c_int
 
 class BHStats(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("visits_total", ctypes.c_int64),
-                ("visits_max", ctypes.c_int64), ("nonfinite", ctypes.c_int64)]
+                ("visits_max", ctypes.c_int64), ("nonfinite", ctypes.c_int64),
+                ("walk_mode", ctypes.c_int64)]
 
 
 # name -> (restype, argtypes); every symbol include/rgsim.h declares.

@@ -176,25 +176,34 @@ def test_fp32_walk_visits_match_reference_walk(orc, name):
 
 @pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3"])
 def test_fp32_walk_modes_agree(name):
-    """The two-lane and wave-coherent walks examine the same nodes as the
-    one-lane walk; the two-lane walk adds two FP32 partial sums at the end,
-    the wave walk keeps each lane's summation order (bitwise equal)."""
+    """Every FP32 walk variant examines the same nodes as the one-lane walk;
+    the split variants add two FP32 partial sums at the end, the wave walks
+    keep each lane's summation order (bitwise equal)."""
     from rgsim import _lib
     pos, mass = CASES[name]
     res = {}
-    for mode in (0, 1, 2, 3):
+    for mode in (0, 1, 2, 3, 4):
         _lib.set_param("walk_mode", mode)
         try:
             bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
             res[mode] = (acc, bh.stats())
         finally:
-            _lib.set_param("walk_mode", 2)
-    (a0, s0), (a1, s1), (a2, s2), (a3, s3) = res[0], res[1], res[2], res[3]
-    for s in (s1, s2, s3):
-        assert s0["visits_total"] == s["visits_total"] and s0["visits_max"] == s["visits_max"]
+            _lib.set_param("walk_mode", -1)
+    (a0, s0) = res[0]
+    for mode in (1, 2, 3, 4):
+        s = res[mode][1]
+        assert s["walk_mode"] == mode
+        assert s0["visits_total"] == s["visits_total"] and s0["visits_max"] == s["visits_max"], mode
+    # the automatic choice below 400k particles is the split wave walk
+    bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
+    assert bh.stats()["walk_mode"] == 4 and np.array_equal(acc, res[4][0])
+    bh, _ = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp64")
+    assert bh.stats()["walk_mode"] == -1
     scale = np.linalg.norm(a0, axis=1).max()
-    assert np.abs(a0 - a1).max() <= 1e-5 * scale
-    assert np.array_equal(a0, a2) and np.array_equal(a0, a3)
+    for mode in (1, 4):  # two partial sums added at the end
+        assert np.abs(a0 - res[mode][0]).max() <= 1e-5 * scale, mode
+    for mode in (2, 3):  # each lane's summation order unchanged
+        assert np.array_equal(a0, res[mode][0]), mode
 
 
 def test_bh_1m_keys_order_topology_bitexact(orc):
