@@ -318,6 +318,71 @@ def bench_bh_1m(steps=5):
     return out
 
 
+def bench_engines(n_gas=65536, n_tree=120000, reps=5):
+    """Self-gravity of the gas-disk and living-galaxy engines: the fused
+    N^2 + analytic potential force (add_grav_ext) on a gas disk in both
+    precisions, the analytic-only and field-add passes, and the tree path
+    with the add fused into the walk against compute + separate add."""
+    import torch
+
+    from rgsim.ic import make_disk_bulge
+    from rgsim.solvers import gravity
+    from rgsim.solvers.barnes_hut import BarnesHut
+    from rgsim.units import G
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    pot = gravity.potential_struct(5.0, 3.0, 0.3, 12.0, 8.0, 0, 0.0)  # the engines' default
+    s = make_disk_bulge(n_disk=n_gas * 7 // 8, n_bulge=n_gas // 8, seed=3)
+    n = s.n
+    pos = torch.from_numpy(s.pos).cuda()
+    mass = torch.from_numpy(s.mass).cuda()
+    acc = torch.zeros_like(pos)
+    out = {"gas_disk": {"n": n, "potential": "Miyamoto-Nagai M_d 5 a 3 b 0.3 + Plummer halo "
+                                             "M_h 12 a_h 8"}}
+    for prec in ("fp64", "fp32c"):
+        ms = timed(lambda: gravity.add_acc_direct_ext(pos, mass, acc, n, G, 0.05 ** 2, pot, prec))
+        gint = n * n / (ms * 1e-3) / 1e9
+        peak = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
+        out["gas_disk"][f"add_grav_ext_{prec}"] = {
+            "ms": ms, "Gint_per_s": gint,
+            "frac_of_peak": gint * FLOP_PER_INTERACTION / 1e3 / peak}
+        log(f"engines: add_grav_ext {prec} N={n}: {ms:.3f} ms, {gint:.0f} Gint/s")
+    ms = timed(lambda: gravity.add_acc_analytic(pos, acc, n, G, pot))
+    out["gas_disk"]["add_analytic"] = {"ms": ms, "GB_per_s": 72 * n / (ms * 1e-3) / 1e9}
+    src = torch.ones_like(pos)
+    ms = timed(lambda: gravity.add_field(acc, src, n))
+    out["gas_disk"]["add_field"] = {"ms": ms, "GB_per_s": 72 * n / (ms * 1e-3) / 1e9}
+
+    t = make_disk_bulge(n_disk=n_tree * 5 // 6, n_bulge=n_tree // 6, seed=1)
+    pt = torch.from_numpy(t.pos).cuda()
+    mt = torch.from_numpy(t.mass).cuda()
+    at = torch.zeros_like(pt)
+    acc_g = torch.empty_like(pt)
+    bh = BarnesHut(t.n, theta=0.5, softening=0.05, precision="fp32")
+
+    def split():
+        bh.compute(pt, mt, acc_g)
+        gravity.add_field(at, acc_g, t.n)
+
+    out["living_galaxy_tree"] = {
+        "n": t.n, "theta": 0.5,
+        "compute_then_add_ms": timed(split),
+        "compute_add_ms": timed(lambda: bh.compute_add(pt, mt, at))}
+    log(f"engines: tree N={t.n}: compute+add {out['living_galaxy_tree']['compute_then_add_ms']:.3f}"
+        f" ms, fused {out['living_galaxy_tree']['compute_add_ms']:.3f} ms")
+    return out
+
+
 def copy_calibration(mib=1024, reps=3):
     """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
     import torch
@@ -368,8 +433,8 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the BH drop-in workload (no direct/adjoint/batch/1M sections)")
-    ap.add_argument("--sections", default="bh,direct,adjoint,batch,bh1m",
-                    help="comma-separated subset of bh,direct,adjoint,batch,bh1m")
+    ap.add_argument("--sections", default="bh,direct,adjoint,batch,bh1m,engines",
+                    help="comma-separated subset of bh,direct,adjoint,batch,bh1m,engines")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -390,7 +455,7 @@ def main():
         sections &= {"bh"}
     if args.no_direct:
         sections.discard("direct")
-    bh = direct = adjoint = batch = bh1m = None
+    bh = direct = adjoint = batch = bh1m = engines = None
     if "bh" in sections:
         log("bench: BH disk+bulge N=120000 ...")
         bh = bench_bh(args.steps, args.warmup)
@@ -407,10 +472,13 @@ def main():
     if "bh1m" in sections:
         log("bench: BH N=1M ...")
         bh1m = bench_bh_1m()
+    if "engines" in sections:
+        log("bench: engine self-gravity ...")
+        engines = bench_engines()
     if bh is None:
         print(json.dumps({"sections": sorted(sections), "direct_plummer_262144": direct,
                           "adjoint_plummer_16384": adjoint, "batch_512_systems": batch,
-                          "bh_disk_bulge_1M": bh1m}))
+                          "bh_disk_bulge_1M": bh1m, "engines": engines}))
         return
     cpu = None
     if not args.no_cpu_baseline:
@@ -443,6 +511,7 @@ def main():
         "adjoint_plummer_16384": adjoint,
         "batch_512_systems": batch,
         "bh_disk_bulge_1M": bh1m,
+        "engines": engines,
     }
     print(json.dumps(line))
 

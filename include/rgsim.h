@@ -79,6 +79,46 @@ int rg_direct_acc_kick(const double *pos, const double *mass, double *acc,
                        double *vel, int64_t n, double G, double eps2,
                        int precision, double dt_half, void *stream);
 
+/* ------------------------------------- self-gravity of the other engines */
+
+/* The analytic galaxy potential the gas-disk and living-galaxy engines add
+ * (their `pot` dict, core/gas_disk_engine.py:220-224): a Miyamoto-Nagai disk
+ * (M_d, a, b), a Plummer halo -- or a Hernquist halo when is_hernquist == 1
+ * -- (M_h, a_h), and a central point mass when smbh_mass > 0.  Evaluated in
+ * FP64 whatever the precision of the self-gravity sum. */
+typedef struct {
+    double M_d, a, b;
+    double M_h, a_h;
+    double smbh_mass;
+    int32_t is_hernquist;
+} rg_potential_t;
+
+/* Replaces living_galaxy_engine.grav_self_direct(pos, mass, acc, n, eps2)
+ * (core/living_galaxy_engine.py:101-113): acc[i] += the rg_direct_acc sum
+ * (same j loop, same precision choices). */
+int rg_direct_acc_add(const double *pos, const double *mass, double *acc, int64_t n,
+                      double G, double eps2, int precision, void *stream);
+
+/* Replaces gas_disk_engine.add_grav_ext(pos, mass, acc, n, eps2, M_d, a, b,
+ * M_h, a_h, is_hernquist, smbh_mass) (core/gas_disk_engine.py:20-51):
+ * acc[i] += (((self-gravity sum + disk) + halo) + point mass), the analytic
+ * terms fused into the force kernel's epilogue. */
+int rg_direct_ext_acc_add(const double *pos, const double *mass, double *acc, int64_t n,
+                          double G, double eps2, const rg_potential_t *pot, int precision,
+                          void *stream);
+
+/* Replaces living_galaxy_engine.grav_analytic (core/living_galaxy_engine.py:
+ * 123-150) and gas_disk_engine.add_analytic (core/gas_disk_engine.py:54-82),
+ * the same arithmetic: acc[i] += ((disk + halo) + point mass).  acc must not
+ * alias pos. */
+int rg_analytic_acc_add(const double *pos, double *acc, int64_t n, double G,
+                        const rg_potential_t *pot, void *stream);
+
+/* Replaces living_galaxy_engine.add_acc(acc, acc_g, n) (core/
+ * living_galaxy_engine.py:116-120) and gas_disk_engine.add_field(acc, src, n)
+ * (core/gas_disk_engine.py:84-89): acc += src.  The two must not overlap. */
+int rg_add_field(double *acc, const double *src, int64_t n, void *stream);
+
 /* ------------------------------------------------------------ integrator */
 
 /* leapfrog.kick (core/integrators/leapfrog.py:11-15): vel += acc * dt_half. */
@@ -125,6 +165,13 @@ int rg_bh_create(int64_t n, double theta, double eps, double G, int morton_bits,
  * the theta walk, all on the device.  Overwrites acc. */
 int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc,
                   void *stream);
+
+/* BarnesHut.compute(pos, mass, acc_g) followed by add_acc(acc, acc_g, n), as
+ * core/living_galaxy_engine.py:1017-1018 and core/gas_disk_engine.py:218-219
+ * call them, with the add fused into the walk's store: acc[p] += a_p, the
+ * same doubles as the two calls, and no acc_g buffer. */
+int rg_bh_compute_add(rg_bh *h, const double *pos, const double *mass, double *acc,
+                      void *stream);
 
 /* Copies of the last compute's tree in the reference's encoding (device
  * pointers): keys_sorted (n) = code (barnes_hut.py:248), order (n),

@@ -32,6 +32,14 @@ Two layers:
   traverse                    theta = 0 reproduces the direct sum exactly in
                               set (every leaf visited), FP64 direct-sum error
                               at theta > 0
+  grav_self_direct,           the direct sum plus the incoming acc, bitwise
+  grav_ext_add                (same j loop); the analytic part as below
+  analytic_add                exact gradients of the Miyamoto-Nagai, Plummer,
+                              Hernquist and point-mass potentials (sympy,
+                              50-digit mpmath); Miyamoto-Nagai a = 0 equals a
+                              Plummer sphere of scale b; in-plane circular
+                              speed closed form; no-op for M_d = M_h = 0
+                              away from the b = z = 0 singular line
   ==========================  ==============================================
 """
 from __future__ import annotations
@@ -97,6 +105,10 @@ def lib() -> ctypes.CDLL:
         L.orc_bh_compute.argtypes = [_dp, _dp, _dp, _i64, _d, _d, _d, ctypes.c_int,
                                      ctypes.c_int, _i64p]
         L.orc_bh_compute.restype = _i64
+        _pot = [_d, _d, _d, _d, _d, ctypes.c_int, _d]
+        L.orc_grav_self_direct.argtypes = [_dp, _dp, _dp, _i64, _d, _d]
+        L.orc_analytic_add.argtypes = [_dp, _dp, _i64, _d] + _pot
+        L.orc_grav_ext_add.argtypes = [_dp, _dp, _dp, _i64, _d, _d] + _pot
         _lib = L
     return _lib
 
@@ -127,6 +139,43 @@ def direct_acc(pos, mass, G, eps2):
     lib().orc_direct_acc(_p(pos, _dp), _p(mass, _dp), _p(acc, _dp), pos.shape[0],
                          float(G), float(eps2))
     return acc
+
+
+def _pot_args(pot):
+    """(M_d, a, b, M_h, a_h, is_hernquist, smbh_mass) of a potential dict
+    shaped like the engines' ``self.pot`` (gas_disk_engine.py:220-224)."""
+    return (float(pot["M_d"]), float(pot["a"]), float(pot["b"]), float(pot["M_h"]),
+            float(pot["a_h"]), int(pot.get("is_hernquist", 0)), float(pot.get("smbh_mass", 0.0)))
+
+
+def grav_self_direct(pos, mass, acc, G, eps2):
+    """living_galaxy_engine.py:101-113 restated in C: returns acc + N^2 sum."""
+    pos = _f64(pos, (-1, 3))
+    mass = _f64(mass, (-1,))
+    out = _f64(acc, (-1, 3)).copy()
+    lib().orc_grav_self_direct(_p(pos, _dp), _p(mass, _dp), _p(out, _dp), pos.shape[0],
+                               float(G), float(eps2))
+    return out
+
+
+def analytic_add(pos, acc, G, pot):
+    """living_galaxy_engine.py:123-150 / gas_disk_engine.py:54-82 restated in
+    C: returns acc + (disk + halo + point mass)."""
+    pos = _f64(pos, (-1, 3))
+    out = _f64(acc, (-1, 3)).copy()
+    lib().orc_analytic_add(_p(pos, _dp), _p(out, _dp), pos.shape[0], float(G), *_pot_args(pot))
+    return out
+
+
+def grav_ext_add(pos, mass, acc, G, eps2, pot):
+    """gas_disk_engine.py:20-51 restated in C: returns acc + (N^2 sum + disk +
+    halo + point mass), summed in that order."""
+    pos = _f64(pos, (-1, 3))
+    mass = _f64(mass, (-1,))
+    out = _f64(acc, (-1, 3)).copy()
+    lib().orc_grav_ext_add(_p(pos, _dp), _p(mass, _dp), _p(out, _dp), pos.shape[0], float(G),
+                           float(eps2), *_pot_args(pot))
+    return out
 
 
 def kinetic_energy(vel, mass):

@@ -17,6 +17,10 @@
  *   orc_build_radix       core/solvers/barnes_hut.py:50-121
  *   orc_aggregate         core/solvers/barnes_hut.py:124-177
  *   orc_traverse          core/solvers/barnes_hut.py:180-217
+ *   orc_grav_self_direct  core/living_galaxy_engine.py:101-113
+ *   orc_analytic_add      core/living_galaxy_engine.py:123-150
+ *                         (= core/gas_disk_engine.py:54-82)
+ *   orc_grav_ext_add      core/gas_disk_engine.py:20-51
  *
  * Arithmetic is FP64 in the reference's operation order.  The file is built
  * with -ffp-contract=off so no multiply-add is fused, matching numpy and the
@@ -456,4 +460,119 @@ int64_t orc_bh_compute(const double *pos, const double *mass, double *acc,
     free(keys); free(code); free(order); free(left); free(right); free(parent);
     free(com); free(nmass); free(nmin); free(nmax);
     return tot;
+}
+
+/* ------------------------------------------------------------------------
+ * Self-gravity and the analytic galaxy potential of the two other engines.
+ * The potential is a Miyamoto-Nagai disk (M_d, a, b), a Plummer halo, or a
+ * Hernquist halo when is_hernquist == 1 (M_h, a_h), and an optional central
+ * point mass smbh_mass.  Taichi's `x ** 2` is x*x, `v.norm()` is
+ * sqrt((x*x + y*y) + z*z), and `x ** (-1.5)` is pow(x, -1.5).
+ */
+
+/* The disk term, then halo and point-mass terms added to av in the order of
+ * gas_disk_engine.py:34-49.  With init != 0 av starts as the disk vector
+ * (living_galaxy_engine.py:134-140); otherwise the disk vector is added to
+ * the self-gravity sum already in av (gas_disk_engine.py:34-39). */
+static void ext_terms(double px, double py, double pz, double grav, double M_d, double a,
+                      double b, double M_h, double a_h, int is_hernquist, double smbh_mass,
+                      double av[3], int init)
+{
+    const double R2 = px * px + py * py;
+    const double S = sqrt(pz * pz + b * b);
+    const double aS = a + S;
+    const double invD3 = pow(R2 + aS * aS, -1.5);
+    const double gmd = -grav * M_d;
+    const double dx = gmd * px * invD3;
+    const double dy = gmd * py * invD3;
+    const double dz = gmd * pz * aS * invD3 / S;
+    if (init) {
+        av[0] = dx; av[1] = dy; av[2] = dz;
+    } else {
+        av[0] += dx; av[1] += dy; av[2] += dz;
+    }
+    if (is_hernquist == 1) {
+        const double r = sqrt(px * px + py * py + pz * pz);
+        const double r_safe = r > 1e-6 ? r : 1e-6;          /* ti.max(r, 1e-6) */
+        const double t = r_safe + a_h;
+        const double c = -(grav * M_h / (t * t * r_safe));
+        av[0] += c * px; av[1] += c * py; av[2] += c * pz;
+    } else {
+        const double rh = R2 + pz * pz + a_h * a_h;
+        const double c = -(grav * M_h * pow(rh, -1.5));
+        av[0] += c * px; av[1] += c * py; av[2] += c * pz;
+    }
+    if (smbh_mass > 0.0) {
+        const double r_safe = sqrt(px * px + py * py + pz * pz) + 1e-6;
+        const double c = -(grav * smbh_mass / (r_safe * r_safe * r_safe));
+        av[0] += c * px; av[1] += c * py; av[2] += c * pz;
+    }
+}
+
+/* Softened sum over all j, the loop of gravity.py:21-25. */
+static void self_sum(const double *pos, const double *mass, int64_t n, int64_t i, double grav,
+                     double eps2, double av[3])
+{
+    const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        const double dx = pos[3 * j] - px;
+        const double dy = pos[3 * j + 1] - py;
+        const double dz = pos[3 * j + 2] - pz;
+        const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+        const double inv = 1.0 / sqrt(r2);
+        const double s = grav * mass[j] * inv * inv * inv;
+        ax += s * dx;
+        ay += s * dy;
+        az += s * dz;
+    }
+    av[0] = ax; av[1] = ay; av[2] = az;
+}
+
+/* living_galaxy_engine.py:101-113: acc[i] += sum_j (softened N^2). */
+void orc_grav_self_direct(const double *pos, const double *mass, double *acc, int64_t n,
+                          double grav, double eps2)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double av[3];
+        self_sum(pos, mass, n, i, grav, eps2, av);
+        acc[3 * i] = acc[3 * i] + av[0];
+        acc[3 * i + 1] = acc[3 * i + 1] + av[1];
+        acc[3 * i + 2] = acc[3 * i + 2] + av[2];
+    }
+}
+
+/* living_galaxy_engine.py:123-150 (and gas_disk_engine.py:54-82, the same
+ * arithmetic): acc[i] += ((disk + halo) + point mass). */
+void orc_analytic_add(const double *pos, double *acc, int64_t n, double grav, double M_d,
+                      double a, double b, double M_h, double a_h, int is_hernquist,
+                      double smbh_mass)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double av[3];
+        ext_terms(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], grav, M_d, a, b, M_h, a_h,
+                  is_hernquist, smbh_mass, av, 1);
+        acc[3 * i] = acc[3 * i] + av[0];
+        acc[3 * i + 1] = acc[3 * i + 1] + av[1];
+        acc[3 * i + 2] = acc[3 * i + 2] + av[2];
+    }
+}
+
+/* gas_disk_engine.py:20-51: acc[i] += (((N^2 sum + disk) + halo) + point mass). */
+void orc_grav_ext_add(const double *pos, const double *mass, double *acc, int64_t n,
+                      double grav, double eps2, double M_d, double a, double b, double M_h,
+                      double a_h, int is_hernquist, double smbh_mass)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double av[3];
+        self_sum(pos, mass, n, i, grav, eps2, av);
+        ext_terms(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], grav, M_d, a, b, M_h, a_h,
+                  is_hernquist, smbh_mass, av, 0);
+        acc[3 * i] = acc[3 * i] + av[0];
+        acc[3 * i + 1] = acc[3 * i + 1] + av[1];
+        acc[3 * i + 2] = acc[3 * i + 2] + av[2];
+    }
 }

@@ -786,11 +786,27 @@ __device__ __forceinline__ int wave_max_i32(int v)
     return v;
 }
 
+// Writes particle p's walk result: acc = a, or acc = acc + a when the caller
+// folds the tree field into an existing acceleration (the "acc += acc_g" of
+// living_galaxy_engine.py:116-120 and gas_disk_engine.py:84-89).
+__device__ __forceinline__ void store_acc(double *__restrict__ acc, int p, double ax, double ay,
+                                          double az, int add)
+{
+    if (add) {
+        ax = acc[3 * p] + ax;
+        ay = acc[3 * p + 1] + ay;
+        az = acc[3 * p + 2] + az;
+    }
+    acc[3 * p] = ax;
+    acc[3 * p + 1] = ay;
+    acc[3 * p + 2] = az;
+}
+
 // FP64: the reference's criterion and arithmetic (barnes_hut.py:196-216),
 // nodes in the reference's pop order via right child / escape links.
 __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restrict__ order,
                          double *__restrict__ acc, int n, double G, double theta2, double eps2,
-                         unsigned long long *__restrict__ stats)
+                         unsigned long long *__restrict__ stats, int add)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -824,9 +840,7 @@ __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restric
         }
         if (node >= 0) atomicAdd(&stats[3], 1ull);  // walk did not terminate in bounds
         const int p = order[t];
-        acc[3 * p] = ax;
-        acc[3 * p + 1] = ay;
-        acc[3 * p + 2] = az;
+        store_acc(acc, p, ax, ay, az, add);
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     const int mx = wave_max_i32(visits);
@@ -843,7 +857,7 @@ __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restric
 __global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict__ recB,
                          const int *__restrict__ pre, const int *__restrict__ order,
                          double *__restrict__ acc, int n, float eps2,
-                         unsigned long long *__restrict__ stats)
+                         unsigned long long *__restrict__ stats, int add)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -875,9 +889,8 @@ __global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict
             }
         }
         const int p = order[t];
-        acc[3 * p] = static_cast<double>(ax);
-        acc[3 * p + 1] = static_cast<double>(ay);
-        acc[3 * p + 2] = static_cast<double>(az);
+        store_acc(acc, p, static_cast<double>(ax), static_cast<double>(ay),
+                  static_cast<double>(az), add);
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     const int mx = wave_max_i32(visits);
@@ -898,7 +911,7 @@ __global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict
 __global__ void k_walk32_pair(const float4 *__restrict__ recA, const int2 *__restrict__ recB,
                               const int *__restrict__ pre, const int *__restrict__ order,
                               double *__restrict__ acc, int n, float eps2,
-                              unsigned long long *__restrict__ stats)
+                              unsigned long long *__restrict__ stats, int add)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int g = b * blockDim.x + threadIdx.x;
@@ -948,9 +961,8 @@ __global__ void k_walk32_pair(const float4 *__restrict__ recA, const int2 *__res
     az += __shfl_xor(az, 1, 64);
     if (active && half == 0) {
         const int p = order[t];
-        acc[3 * p] = static_cast<double>(ax);
-        acc[3 * p + 1] = static_cast<double>(ay);
-        acc[3 * p + 2] = static_cast<double>(az);
+        store_acc(acc, p, static_cast<double>(ax), static_cast<double>(ay),
+                  static_cast<double>(az), add);
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     // per-particle visits = the pair's sum
@@ -976,7 +988,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave(const float4 *__restrict__ 
                                                      const int *__restrict__ pre,
                                                      const int *__restrict__ order,
                                                      double *__restrict__ acc, int n, float eps2,
-                                                     unsigned long long *__restrict__ stats)
+                                                     unsigned long long *__restrict__ stats,
+                                                     int add)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -1039,9 +1052,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave(const float4 *__restrict__ 
     }
     if (active) {
         const int p = order[t];
-        acc[3 * p] = static_cast<double>(ax);
-        acc[3 * p + 1] = static_cast<double>(ay);
-        acc[3 * p + 2] = static_cast<double>(az);
+        store_acc(acc, p, static_cast<double>(ax), static_cast<double>(ay),
+                  static_cast<double>(az), add);
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     const int mx = wave_max_i32(visits);
@@ -1062,7 +1074,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__
                                                       const int *__restrict__ order,
                                                       double *__restrict__ acc, int n,
                                                       float eps2,
-                                                      unsigned long long *__restrict__ stats)
+                                                      unsigned long long *__restrict__ stats,
+                                                      int add)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -1147,9 +1160,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__
     }
     if (active) {
         const int p = order[t];
-        acc[3 * p] = static_cast<double>(ax);
-        acc[3 * p + 1] = static_cast<double>(ay);
-        acc[3 * p + 2] = static_cast<double>(az);
+        store_acc(acc, p, static_cast<double>(ax), static_cast<double>(ay),
+                  static_cast<double>(az), add);
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     const int mx = wave_max_i32(visits);
@@ -1171,7 +1183,7 @@ __global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__
 __global__ __launch_bounds__(256) void k_walk32_wave_split(
     const float4 *__restrict__ recA, const int2 *__restrict__ recB, const int *__restrict__ pre,
     const int *__restrict__ order, double *__restrict__ acc, int n, float eps2,
-    unsigned long long *__restrict__ stats)
+    unsigned long long *__restrict__ stats, int add)
 {
     __shared__ float s_part[2][64][3];  // wave 1's partial sums, per group
     __shared__ int s_vis[2][64];
@@ -1262,9 +1274,9 @@ __global__ __launch_bounds__(256) void k_walk32_wave_split(
         pv = visits + s_vis[grp][lane];
         if (active) {
             const int p = order[t];
-            acc[3 * p] = static_cast<double>(ax + s_part[grp][lane][0]);
-            acc[3 * p + 1] = static_cast<double>(ay + s_part[grp][lane][1]);
-            acc[3 * p + 2] = static_cast<double>(az + s_part[grp][lane][2]);
+            store_acc(acc, p, static_cast<double>(ax + s_part[grp][lane][0]),
+                      static_cast<double>(ay + s_part[grp][lane][1]),
+                      static_cast<double>(az + s_part[grp][lane][2]), add);
         }
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
@@ -1433,8 +1445,10 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     return RG_OK;
 }
 
-extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc,
-                             void *stream)
+namespace {
+
+int bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc, int add,
+               void *stream)
 {
     RG_REQUIRE(h, RG_ERR_ARG, "null handle");
     RG_REQUIRE(pos && mass && acc, RG_ERR_ARG, "null pointer argument");
@@ -1533,7 +1547,8 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
     const double eps2 = h->eps * h->eps;
     h->walk_used = -1;
     if (fp64) {
-        k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, acc, n, h->G, theta2, eps2, h->stats);
+        k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, acc, n, h->G, theta2, eps2, h->stats,
+                                    add);
     } else {
         // Below ~400k particles the walk has too few waves to hide its step
         // latency, and splitting each group over two waves pays; above, the
@@ -1543,22 +1558,22 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
         if (mode == 4) {
             k_walk32_wave_split<<<(n + 127) / 128, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
                                                                  acc, n, static_cast<float>(eps2),
-                                                                 h->stats);
+                                                                 h->stats, add);
         } else if (mode == 3) {
             k_walk32_wave2<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
                                                             acc, n, static_cast<float>(eps2),
-                                                            h->stats);
+                                                            h->stats, add);
         } else if (mode == 2) {
             k_walk32_wave<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
                                                            acc, n, static_cast<float>(eps2),
-                                                           h->stats);
+                                                           h->stats, add);
         } else if (mode == 1) {
             const int wg2 = (2 * n + wb - 1) / wb;
             k_walk32_pair<<<wg2, wb, 0, st>>>(h->recA, h->recB, h->pre, order, acc, n,
-                                              static_cast<float>(eps2), h->stats);
+                                              static_cast<float>(eps2), h->stats, add);
         } else {
             k_walk32<<<wg, wb, 0, st>>>(h->recA, h->recB, h->pre, order, acc, n,
-                                        static_cast<float>(eps2), h->stats);
+                                        static_cast<float>(eps2), h->stats, add);
         }
     }
     RG_CHECK_LAUNCH();
@@ -1568,6 +1583,20 @@ extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, do
     h->last_stream = st;
     h->computed = true;
     return RG_OK;
+}
+
+}  // namespace
+
+extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc,
+                             void *stream)
+{
+    return bh_compute(h, pos, mass, acc, 0, stream);
+}
+
+extern "C" int rg_bh_compute_add(rg_bh *h, const double *pos, const double *mass, double *acc,
+                                 void *stream)
+{
+    return bh_compute(h, pos, mass, acc, 1, stream);
 }
 
 extern "C" int rg_bh_set_timing(rg_bh *h, int enable)

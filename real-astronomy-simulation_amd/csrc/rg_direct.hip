@@ -33,6 +33,69 @@ __device__ __forceinline__ void split_df(double x, float &hi, float &lo)
     lo = static_cast<float>(x - static_cast<double>(hi));
 }
 
+// What the kernel does with an i-particle's sum once the j loop is done.
+enum EpiMode { kStore = 0, kAdd = 1, kAddExt = 2 };
+
+struct Epilogue {
+    int mode;                                   // EpiMode
+    int is_hernquist;
+    double M_d, a, b, M_h, a_h, smbh_mass;      // analytic potential (kAddExt)
+};
+
+Epilogue make_epilogue(int mode, const rg_potential_t *pot)
+{
+    Epilogue e{};
+    e.mode = mode;
+    if (pot) {
+        e.M_d = pot->M_d; e.a = pot->a; e.b = pot->b;
+        e.M_h = pot->M_h; e.a_h = pot->a_h; e.smbh_mass = pot->smbh_mass;
+        e.is_hernquist = pot->is_hernquist;
+    }
+    return e;
+}
+
+// The analytic galaxy potential of core/gas_disk_engine.py:34-49 (the same
+// arithmetic as core/living_galaxy_engine.py:134-149), FP64, in that order:
+// a Miyamoto-Nagai disk, then a Hernquist (is_hernquist == 1) or Plummer
+// halo, then a central point mass when smbh_mass > 0.  With init the disk
+// vector starts the sum (living_galaxy_engine.py:138); otherwise it is
+// added to the self-gravity sum already in av (gas_disk_engine.py:38).
+// Taichi's `x ** 2` is x*x, `v.norm()` sqrt((x*x + y*y) + z*z), and
+// `x ** (-1.5)` pow(x, -1.5).
+__device__ void ext_terms(double px, double py, double pz, double G, const Epilogue &e,
+                          double av[3], bool init)
+{
+    const double R2 = px * px + py * py;
+    const double S = sqrt(pz * pz + e.b * e.b);
+    const double aS = e.a + S;
+    const double invD3 = pow(R2 + aS * aS, -1.5);
+    const double gmd = -G * e.M_d;
+    const double dx = gmd * px * invD3;
+    const double dy = gmd * py * invD3;
+    const double dz = gmd * pz * aS * invD3 / S;
+    if (init) {
+        av[0] = dx; av[1] = dy; av[2] = dz;
+    } else {
+        av[0] += dx; av[1] += dy; av[2] += dz;
+    }
+    double c;
+    if (e.is_hernquist == 1) {
+        const double r = sqrt(px * px + py * py + pz * pz);
+        const double r_safe = fmax(r, 1e-6);
+        const double t = r_safe + e.a_h;
+        c = -(G * e.M_h / (t * t * r_safe));
+    } else {
+        const double rh = R2 + pz * pz + e.a_h * e.a_h;
+        c = -(G * e.M_h * pow(rh, -1.5));
+    }
+    av[0] += c * px; av[1] += c * py; av[2] += c * pz;
+    if (e.smbh_mass > 0.0) {
+        const double r_safe = sqrt(px * px + py * py + pz * pz) + 1e-6;
+        const double cb = -(G * e.smbh_mass / (r_safe * r_safe * r_safe));
+        av[0] += cb * px; av[1] += cb * py; av[2] += cb * pz;
+    }
+}
+
 // Knuth's TwoSum: s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
 {
@@ -48,7 +111,7 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
                                                 const double *__restrict__ mass,
                                                 double *__restrict__ acc,
                                                 double *__restrict__ vel, int n, double G,
-                                                double eps2, double dt_half)
+                                                double eps2, double dt_half, Epilogue epi)
 {
     using T4 = typename std::conditional<MODE == 0 || MODE == 3, double4, float4>::type;
     __shared__ T4 s_hi[kBS];
@@ -225,6 +288,16 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
             ay = static_cast<double>(sy[k]) + static_cast<double>(cy[k]);
             az = static_cast<double>(sz[k]) + static_cast<double>(cz[k]);
         }
+        if (epi.mode != kStore) {  // uniform across the grid
+            if (epi.mode == kAddExt) {
+                double av[3] = {ax, ay, az};
+                ext_terms(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], G, epi, av, false);
+                ax = av[0]; ay = av[1]; az = av[2];
+            }
+            ax = acc[3 * i] + ax;
+            ay = acc[3 * i + 1] + ay;
+            az = acc[3 * i + 2] + az;
+        }
         acc[3 * i] = ax;
         acc[3 * i + 1] = ay;
         acc[3 * i + 2] = az;
@@ -238,7 +311,7 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
 
 template <int MODE>
 int launch_direct(const double *pos, const double *mass, double *acc, double *vel, int n,
-                  double G, double eps2, double dt_half, hipStream_t st)
+                  double G, double eps2, double dt_half, const Epilogue &epi, hipStream_t st)
 {
     // One i-particle per lane measured fastest in every precision (the
     // kernels are VALU-bound; more i per lane only adds register pressure).
@@ -247,13 +320,16 @@ int launch_direct(const double *pos, const double *mass, double *acc, double *ve
     const int grid = (n + per_block - 1) / per_block;
     switch (ipt) {
     case 1:
-        k_direct<MODE, 1><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half);
+        k_direct<MODE, 1><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
+                                                  epi);
         break;
     case 2:
-        k_direct<MODE, 2><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half);
+        k_direct<MODE, 2><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
+                                                  epi);
         break;
     case 4:
-        k_direct<MODE, 4><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half);
+        k_direct<MODE, 4><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
+                                                  epi);
         break;
     default:
         rg::set_error("direct_ipt must be 1, 2 or 4 (got %d)", ipt);
@@ -265,7 +341,7 @@ int launch_direct(const double *pos, const double *mass, double *acc, double *ve
 
 int direct_common(const double *pos, const double *mass, double *acc, double *vel,
                   int64_t n, double G, double eps2, int precision, double dt_half,
-                  void *stream)
+                  const Epilogue &epi, void *stream)
 {
     RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY,
                "n=%lld outside [0, 2^29]", static_cast<long long>(n));
@@ -283,11 +359,12 @@ int direct_common(const double *pos, const double *mass, double *acc, double *ve
     hipStream_t st = rg::as_stream(stream);
     const int ni = static_cast<int>(n);
     switch (precision) {
-    case RG_FP64: return launch_direct<0>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
-    case RG_FP32: return launch_direct<1>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
+    case RG_FP64: return launch_direct<0>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
+    case RG_FP32: return launch_direct<1>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
     case RG_FP32C:
-        if (g_fp32c_d64) return launch_direct<3>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
-        return launch_direct<2>(pos, mass, acc, vel, ni, G, eps2, dt_half, st);
+        if (g_fp32c_d64)
+            return launch_direct<3>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
+        return launch_direct<2>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
     default: break;
     }
     rg::set_error("unknown precision %d", precision);
@@ -314,6 +391,27 @@ __global__ void k_kick_drift(double *__restrict__ x, double *__restrict__ v,
         const double vk = v[k] + a[k] * h;
         v[k] = vk;
         x[k] = x[k] + vk * dt;
+    }
+}
+
+// living_galaxy_engine.py:116-120 add_acc / gas_disk_engine.py:84-89 add_field.
+__global__ void k_add_field(double *__restrict__ acc, const double *__restrict__ src, int m)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+        acc[k] = acc[k] + src[k];
+}
+
+// living_galaxy_engine.py:123-150 grav_analytic / gas_disk_engine.py:54-82
+// add_analytic: acc[i] += ((disk + halo) + point mass).
+__global__ void k_analytic_add(const double *__restrict__ pos, double *__restrict__ acc, int n,
+                               double G, Epilogue epi)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double av[3];
+        ext_terms(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], G, epi, av, true);
+        acc[3 * i] = acc[3 * i] + av[0];
+        acc[3 * i + 1] = acc[3 * i + 1] + av[1];
+        acc[3 * i + 2] = acc[3 * i + 2] + av[2];
     }
 }
 
@@ -415,7 +513,52 @@ int rg::rg_direct_set_ipt_impl(int64_t v)
 extern "C" int rg_direct_acc(const double *pos, const double *mass, double *acc, int64_t n,
                              double G, double eps2, int precision, void *stream)
 {
-    return direct_common(pos, mass, acc, nullptr, n, G, eps2, precision, 0.0, stream);
+    return direct_common(pos, mass, acc, nullptr, n, G, eps2, precision, 0.0,
+                         make_epilogue(kStore, nullptr), stream);
+}
+
+extern "C" int rg_direct_acc_add(const double *pos, const double *mass, double *acc, int64_t n,
+                                 double G, double eps2, int precision, void *stream)
+{
+    return direct_common(pos, mass, acc, nullptr, n, G, eps2, precision, 0.0,
+                         make_epilogue(kAdd, nullptr), stream);
+}
+
+extern "C" int rg_direct_ext_acc_add(const double *pos, const double *mass, double *acc,
+                                     int64_t n, double G, double eps2,
+                                     const rg_potential_t *pot, int precision, void *stream)
+{
+    RG_REQUIRE(pot, RG_ERR_ARG, "null potential");
+    return direct_common(pos, mass, acc, nullptr, n, G, eps2, precision, 0.0,
+                         make_epilogue(kAddExt, pot), stream);
+}
+
+extern "C" int rg_analytic_acc_add(const double *pos, double *acc, int64_t n, double G,
+                                   const rg_potential_t *pot, void *stream)
+{
+    RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "n out of range");
+    if (n == 0) return RG_OK;
+    RG_REQUIRE(pos && acc && pot, RG_ERR_ARG, "null pointer argument");
+    const size_t b3 = sizeof(double) * 3 * static_cast<size_t>(n);
+    RG_REQUIRE(!rg_overlap(acc, b3, pos, b3), RG_ERR_ALIAS, "acc aliases pos");
+    const int ni = static_cast<int>(n);
+    k_analytic_add<<<elementwise_grid(ni), 256, 0, rg::as_stream(stream)>>>(
+        pos, acc, ni, G, make_epilogue(kAddExt, pot));
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
+extern "C" int rg_add_field(double *acc, const double *src, int64_t n, void *stream)
+{
+    RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "n out of range");
+    if (n == 0) return RG_OK;
+    RG_REQUIRE(acc && src, RG_ERR_ARG, "null pointer argument");
+    const size_t b3 = sizeof(double) * 3 * static_cast<size_t>(n);
+    RG_REQUIRE(!rg_overlap(acc, b3, src, b3), RG_ERR_ALIAS, "acc and src overlap");
+    const int m = static_cast<int>(3 * n);
+    k_add_field<<<elementwise_grid(m), 256, 0, rg::as_stream(stream)>>>(acc, src, m);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
 }
 
 extern "C" int rg_direct_acc_kick(const double *pos, const double *mass, double *acc,
@@ -423,7 +566,8 @@ extern "C" int rg_direct_acc_kick(const double *pos, const double *mass, double 
                                   int precision, double dt_half, void *stream)
 {
     RG_REQUIRE(vel || n == 0, RG_ERR_ARG, "null vel");
-    return direct_common(pos, mass, acc, vel, n, G, eps2, precision, dt_half, stream);
+    return direct_common(pos, mass, acc, vel, n, G, eps2, precision, dt_half,
+                         make_epilogue(kStore, nullptr), stream);
 }
 
 extern "C" int rg_kick(double *vel, const double *acc, int64_t n, double dt_half, void *stream)

@@ -34,6 +34,13 @@ class BHStats(ctypes.Structure):
                 ("walk_mode", ctypes.c_int64)]
 
 
+class Potential(ctypes.Structure):
+    """rg_potential_t: the engines' analytic disk + halo + point-mass potential."""
+    _fields_ = [("M_d", ctypes.c_double), ("a", ctypes.c_double), ("b", ctypes.c_double),
+                ("M_h", ctypes.c_double), ("a_h", ctypes.c_double),
+                ("smbh_mass", ctypes.c_double), ("is_hernquist", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes); every symbol include/rgsim.h declares.
 SIGNATURES = {
     "rg_version": (_i, []),
@@ -41,6 +48,11 @@ SIGNATURES = {
     "rg_set_param": (_i, [ctypes.c_char_p, _i64]),
     "rg_direct_acc": (_i, [_vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
     "rg_direct_acc_kick": (_i, [_vp, _vp, _vp, _vp, _i64, _d, _d, _i, _d, _vp]),
+    "rg_direct_acc_add": (_i, [_vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
+    "rg_direct_ext_acc_add": (_i, [_vp, _vp, _vp, _i64, _d, _d, ctypes.POINTER(Potential), _i,
+                                   _vp]),
+    "rg_analytic_acc_add": (_i, [_vp, _vp, _i64, _d, ctypes.POINTER(Potential), _vp]),
+    "rg_add_field": (_i, [_vp, _vp, _i64, _vp]),
     "rg_kick": (_i, [_vp, _vp, _i64, _d, _vp]),
     "rg_drift": (_i, [_vp, _vp, _i64, _d, _vp]),
     "rg_kick_drift": (_i, [_vp, _vp, _vp, _i64, _d, _d, _vp]),
@@ -48,6 +60,7 @@ SIGNATURES = {
     "rg_morton_codes": (_i, [_vp, _i64, _i, _vp, _vp]),
     "rg_bh_create": (_i, [_i64, _d, _d, _d, _i, _i, ctypes.POINTER(_vp)]),
     "rg_bh_compute": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "rg_bh_compute_add": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "rg_bh_export_tree": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rg_bh_export_nodes": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rg_bh_stats": (_i, [_vp, ctypes.POINTER(BHStats)]),

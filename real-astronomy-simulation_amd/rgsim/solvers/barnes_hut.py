@@ -66,6 +66,16 @@ class BarnesHut:
                                              acc.data_ptr(), _lib.stream_ptr(pos)),
                    "rg_bh_compute")
 
+    def compute_add(self, pos, mass, acc) -> None:
+        """acc += the treecode acceleration: compute(pos, mass, acc_g) then
+        acc += acc_g, as living_galaxy_engine.py:1017-1018 and
+        gas_disk_engine.py:218-219 do, with the add fused into the walk."""
+        n = self.n
+        vec3(pos, n, "pos"), vec1(mass, n, "mass"), vec3(acc, n, "acc")
+        _lib.check(_lib.load().rg_bh_compute_add(self._h, pos.data_ptr(), mass.data_ptr(),
+                                                 acc.data_ptr(), _lib.stream_ptr(pos)),
+                   "rg_bh_compute_add")
+
     def export_tree(self):
         """(keys_sorted, order, left, right, parent) of the last compute, as
         device tensors in the reference's encoding."""

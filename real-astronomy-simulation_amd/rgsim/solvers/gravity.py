@@ -5,11 +5,21 @@ core/solvers/gravity.py:13-57 over torch device buffers.
     kinetic_energy(vel, mass, n)                         gravity.py:29-36
     potential_energy(pos, mass, n, grav, eps2)           gravity.py:39-57
 
+and the accumulate forms the gas-disk and living-galaxy engines use
+(core/gas_disk_engine.py:20-89, core/living_galaxy_engine.py:101-150):
+
+    add_acc_direct        acc += self-gravity sum
+    add_acc_direct_ext    acc += self-gravity sum + analytic potential, fused
+    add_acc_analytic      acc += analytic potential
+    add_field             acc += src
+
 ``pos``/``acc`` are (n,3) and ``mass`` (n,) contiguous float64 CUDA tensors.
 ``precision`` selects the pair arithmetic: "fp64" (the reference's),
 "fp32", or "fp32c" (FP32 on hi+lo positions with compensated sums).
 """
 from __future__ import annotations
+
+import ctypes
 
 import torch
 
@@ -38,6 +48,61 @@ def compute_acc_direct_kick(pos, mass, acc, vel, n, grav, eps2, dt_half, precisi
                                               _lib.precision_code(precision), float(dt_half),
                                               _lib.stream_ptr(pos)),
                "rg_direct_acc_kick")
+
+
+def potential_struct(M_d, a, b, M_h, a_h, is_hernquist, smbh_mass):
+    """rg_potential_t from the engines' potential arguments
+    (gas_disk_engine.py:21-23)."""
+    return _lib.Potential(float(M_d), float(a), float(b), float(M_h), float(a_h),
+                          float(smbh_mass), int(is_hernquist))
+
+
+def add_acc_direct(pos, mass, acc, n, grav, eps2, precision="fp64"):
+    """acc += G sum_j m_j d_ij / (|d_ij|^2 + eps2)^{3/2}
+    (living_galaxy_engine.py:101-113)."""
+    n = int(n)
+    vec3(pos, n, "pos"), vec1(mass, n, "mass"), vec3(acc, n, "acc")
+    _lib.require_gpu()
+    _lib.check(_lib.load().rg_direct_acc_add(pos.data_ptr(), mass.data_ptr(), acc.data_ptr(),
+                                             n, float(grav), float(eps2),
+                                             _lib.precision_code(precision),
+                                             _lib.stream_ptr(pos)),
+               "rg_direct_acc_add")
+
+
+def add_acc_direct_ext(pos, mass, acc, n, grav, eps2, pot, precision="fp64"):
+    """acc += self-gravity sum + analytic potential (gas_disk_engine.py:20-51);
+    ``pot`` is an rg_potential_t (``potential_struct``)."""
+    n = int(n)
+    vec3(pos, n, "pos"), vec1(mass, n, "mass"), vec3(acc, n, "acc")
+    _lib.require_gpu()
+    _lib.check(_lib.load().rg_direct_ext_acc_add(pos.data_ptr(), mass.data_ptr(),
+                                                 acc.data_ptr(), n, float(grav), float(eps2),
+                                                 ctypes.byref(pot),
+                                                 _lib.precision_code(precision),
+                                                 _lib.stream_ptr(pos)),
+               "rg_direct_ext_acc_add")
+
+
+def add_acc_analytic(pos, acc, n, grav, pot):
+    """acc += analytic disk + halo + point-mass acceleration
+    (living_galaxy_engine.py:123-150, gas_disk_engine.py:54-82)."""
+    n = int(n)
+    vec3(pos, n, "pos"), vec3(acc, n, "acc")
+    _lib.require_gpu()
+    _lib.check(_lib.load().rg_analytic_acc_add(pos.data_ptr(), acc.data_ptr(), n, float(grav),
+                                               ctypes.byref(pot), _lib.stream_ptr(pos)),
+               "rg_analytic_acc_add")
+
+
+def add_field(acc, src, n):
+    """acc += src (living_galaxy_engine.py:116-120, gas_disk_engine.py:84-89)."""
+    n = int(n)
+    vec3(acc, n, "acc"), vec3(src, n, "src")
+    _lib.require_gpu()
+    _lib.check(_lib.load().rg_add_field(acc.data_ptr(), src.data_ptr(), n,
+                                        _lib.stream_ptr(acc)),
+               "rg_add_field")
 
 
 def energy_terms(pos, vel, mass, n, grav, eps2, with_pe=True):
