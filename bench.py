@@ -36,7 +36,8 @@ FLOP_PER_INTERACTION = 20
 WALK_RECORD_BYTES = 32          # FP32 node record fetched per visit
 # FP32 walk variant (rg_bh_stats walk_mode) -> the kernel it launches
 WALK_KERNELS = {0: "k_walk32", 1: "k_walk32_pair", 2: "k_walk32_wave", 3: "k_walk32_wave2",
-                4: "k_walk32_wave_split", -1: "k_walk64"}
+                4: "k_walk32_wave_split<2>", 5: "k_walk32_wave_split<4>",
+                6: "k_walk32_chunked<4>", -1: "k_walk64"}
 WALK_PARTICLE_BYTES = 16 + 4 + 24  # own leaf record + order index + FP64 acceleration
 
 
@@ -72,23 +73,32 @@ def bench_bh(steps, warmup):
     for _ in range(warmup):
         eng.step(dt)
     torch.cuda.synchronize()
-    eng._bh.set_timing(True)
+    # the timed steps run without phase events (each event adds a gap)
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.step(dt)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    wall = t1 - t0
+    # phase times and the walk's launch duration from HIP events on the
+    # compute stream, over further steps; visits averaged over the same steps
+    phase_steps = min(steps, 20)
+    eng._bh.set_timing(True)
+    visits = 0
+    for _ in range(phase_steps):
+        eng.step(dt)
+        visits += eng._bh.stats()["visits_total"]
     tm = eng._bh.timing()
     eng._bh.set_timing(False)
     st_last = eng._bh.stats()
-    wall = t1 - t0
+    visits /= phase_steps
     walk_ms = tm["walk_ms"] / max(tm["count"], 1)
-    visits = st_last["visits_total"]
     walk_bytes = visits * WALK_RECORD_BYTES + n * WALK_PARTICLE_BYTES
     achieved = walk_bytes / (walk_ms * 1e-3) / 1e9
     kernel = WALK_KERNELS[st_last["walk_mode"]]
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"{kernel}_diskbulge120k_pmc.json")
+    fname = kernel.replace("<", "").replace(">", "")
+    pmc = os.path.join(ROOT, "profiles", f"{fname}_diskbulge120k_pmc.json")
     if os.path.exists(pmc):  # rocprofv3 FETCH_SIZE / WRITE_SIZE of this kernel, copy-scaled
         with open(pmc) as f:
             rec = json.load(f)

@@ -54,11 +54,12 @@ const char *rg_last_error(void);
  *   "direct_ipt"  i-particles per lane in the direct kernels: 0 (heuristic),
  *                 1, 2 or 4
  *   "walk_block"  threads per block of the BH walk: 64, 128, 256 or 512
- *   "walk_mode"   FP32 BH walk: -1 auto (default: 4 below 400k particles,
- *                 else 2), 0 one lane per particle, 1 two lanes per
- *                 particle, 2 one wave walking together, 3 the same
- *                 advancing two slots per step, 4 the wave walk split over
- *                 two waves per particle group
+ *   "walk_mode"   FP32 BH walk: -1 auto (default: 6), 0 one lane per
+ *                 particle, 1 two lanes per particle, 2 one wave walking
+ *                 together, 3 the same advancing two slots per step, 4 / 5
+ *                 the wave walk split over two / four waves per particle
+ *                 group, 6 the wave walk over far-field chunks of the
+ *                 preorder that four waves per group take in turn
  * Process-wide; takes effect at the next launch. */
 int rg_set_param(const char *name, int64_t value);
 
@@ -166,6 +167,13 @@ int rg_bh_create(int64_t n, double theta, double eps, double G, int morton_bits,
 int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc,
                   void *stream);
 
+/* rg_bh_compute followed by leapfrog.kick(vel, acc, n, dt_half)
+ * (core/integrators/leapfrog.py:11-15) fused into the walk's store: the
+ * closing half-kick of Engine.step (core/engine.py:113-114), bit-identical
+ * to the two calls. */
+int rg_bh_compute_kick(rg_bh *h, const double *pos, const double *mass, double *acc,
+                       double *vel, double dt_half, void *stream);
+
 /* BarnesHut.compute(pos, mass, acc_g) followed by add_acc(acc, acc_g, n), as
  * core/living_galaxy_engine.py:1017-1018 and core/gas_disk_engine.py:218-219
  * call them, with the add fused into the walk's store: acc[p] += a_p, the
@@ -191,7 +199,7 @@ typedef struct {
     int64_t visits_total;   /* nodes examined by the walk, all particles */
     int64_t visits_max;     /* max nodes examined by one particle */
     int64_t nonfinite;      /* non-finite input coordinates seen */
-    int64_t walk_mode;      /* FP32 walk variant used (0..4, see "walk_mode"); -1 FP64 */
+    int64_t walk_mode;      /* FP32 walk variant used (0..6, see "walk_mode"); -1 FP64 */
 } rg_bh_stats_t;
 
 /* Statistics of the last compute.  Synchronises that compute's stream. */

@@ -50,7 +50,8 @@ constexpr int kWalkBSDefault = 256;
 
 int g_walk_bs = kWalkBSDefault;
 // FP32 walk: -1 auto, 0 one lane, 1 two lanes, 2 wave-coherent, 3 wave with two
-// slots per step, 4 wave-coherent split over two waves per particle group
+// slots per step, 4 / 5 wave-coherent split over two / four waves per particle
+// group, 6 wave-coherent over far-field chunks taken by four waves
 int g_walk_mode = -1;
 
 struct alignas(16) WalkRec64 {
@@ -786,13 +787,22 @@ __device__ __forceinline__ int wave_max_i32(int v)
     return v;
 }
 
-// Writes particle p's walk result: acc = a, or acc = acc + a when the caller
-// folds the tree field into an existing acceleration (the "acc += acc_g" of
-// living_galaxy_engine.py:116-120 and gas_disk_engine.py:84-89).
-__device__ __forceinline__ void store_acc(double *__restrict__ acc, int p, double ax, double ay,
-                                          double az, int add)
+// Where a particle's walk result goes: acc = a, or acc = acc + a when the
+// caller folds the tree field into an existing acceleration (the
+// "acc += acc_g" of living_galaxy_engine.py:116-120 and gas_disk_engine.py:
+// 84-89); then, when vel is set, the closing half-kick of the KDK step,
+// vel += acc * dt_half (leapfrog.py:14, engine.py:113-114).
+struct AccOut {
+    double *acc;
+    double *vel;
+    double dt_half;
+    int add;
+};
+
+__device__ __forceinline__ void store_acc(const AccOut &o, int p, double ax, double ay, double az)
 {
-    if (add) {
+    double *__restrict__ acc = o.acc;
+    if (o.add) {
         ax = acc[3 * p] + ax;
         ay = acc[3 * p + 1] + ay;
         az = acc[3 * p + 2] + az;
@@ -800,13 +810,19 @@ __device__ __forceinline__ void store_acc(double *__restrict__ acc, int p, doubl
     acc[3 * p] = ax;
     acc[3 * p + 1] = ay;
     acc[3 * p + 2] = az;
+    if (o.vel) {
+        double *__restrict__ v = o.vel;
+        v[3 * p] = v[3 * p] + ax * o.dt_half;
+        v[3 * p + 1] = v[3 * p + 1] + ay * o.dt_half;
+        v[3 * p + 2] = v[3 * p + 2] + az * o.dt_half;
+    }
 }
 
 // FP64: the reference's criterion and arithmetic (barnes_hut.py:196-216),
 // nodes in the reference's pop order via right child / escape links.
 __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restrict__ order,
-                         double *__restrict__ acc, int n, double G, double theta2, double eps2,
-                         unsigned long long *__restrict__ stats, int add)
+                         AccOut out, int n, double G, double theta2, double eps2,
+                         unsigned long long *__restrict__ stats)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -840,7 +856,7 @@ __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restric
         }
         if (node >= 0) atomicAdd(&stats[3], 1ull);  // walk did not terminate in bounds
         const int p = order[t];
-        store_acc(acc, p, ax, ay, az, add);
+        store_acc(out, p, ax, ay, az);
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     const int mx = wave_max_i32(visits);
@@ -856,8 +872,8 @@ __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restric
 // most 2n-1 steps.
 __global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict__ recB,
                          const int *__restrict__ pre, const int *__restrict__ order,
-                         double *__restrict__ acc, int n, float eps2,
-                         unsigned long long *__restrict__ stats, int add)
+                         AccOut out, int n, float eps2,
+                         unsigned long long *__restrict__ stats)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -889,8 +905,8 @@ __global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict
             }
         }
         const int p = order[t];
-        store_acc(acc, p, static_cast<double>(ax), static_cast<double>(ay),
-                  static_cast<double>(az), add);
+        store_acc(out, p, static_cast<double>(ax), static_cast<double>(ay),
+                  static_cast<double>(az));
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     const int mx = wave_max_i32(visits);
@@ -910,8 +926,8 @@ __global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict
 // owner, so the accepted set and visit counts are unchanged.
 __global__ void k_walk32_pair(const float4 *__restrict__ recA, const int2 *__restrict__ recB,
                               const int *__restrict__ pre, const int *__restrict__ order,
-                              double *__restrict__ acc, int n, float eps2,
-                              unsigned long long *__restrict__ stats, int add)
+                              AccOut out, int n, float eps2,
+                              unsigned long long *__restrict__ stats)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int g = b * blockDim.x + threadIdx.x;
@@ -961,8 +977,8 @@ __global__ void k_walk32_pair(const float4 *__restrict__ recA, const int2 *__res
     az += __shfl_xor(az, 1, 64);
     if (active && half == 0) {
         const int p = order[t];
-        store_acc(acc, p, static_cast<double>(ax), static_cast<double>(ay),
-                  static_cast<double>(az), add);
+        store_acc(out, p, static_cast<double>(ax), static_cast<double>(ay),
+                  static_cast<double>(az));
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     // per-particle visits = the pair's sum
@@ -987,9 +1003,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave(const float4 *__restrict__ 
                                                      const int2 *__restrict__ recB,
                                                      const int *__restrict__ pre,
                                                      const int *__restrict__ order,
-                                                     double *__restrict__ acc, int n, float eps2,
-                                                     unsigned long long *__restrict__ stats,
-                                                     int add)
+                                                     AccOut out, int n, float eps2,
+                                                     unsigned long long *__restrict__ stats)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -1052,8 +1067,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave(const float4 *__restrict__ 
     }
     if (active) {
         const int p = order[t];
-        store_acc(acc, p, static_cast<double>(ax), static_cast<double>(ay),
-                  static_cast<double>(az), add);
+        store_acc(out, p, static_cast<double>(ax), static_cast<double>(ay),
+                  static_cast<double>(az));
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     const int mx = wave_max_i32(visits);
@@ -1072,10 +1087,9 @@ __global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__
                                                       const int2 *__restrict__ recB,
                                                       const int *__restrict__ pre,
                                                       const int *__restrict__ order,
-                                                      double *__restrict__ acc, int n,
+                                                      AccOut out, int n,
                                                       float eps2,
-                                                      unsigned long long *__restrict__ stats,
-                                                      int add)
+                                                      unsigned long long *__restrict__ stats)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
@@ -1160,8 +1174,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__
     }
     if (active) {
         const int p = order[t];
-        store_acc(acc, p, static_cast<double>(ax), static_cast<double>(ay),
-                  static_cast<double>(az), add);
+        store_acc(out, p, static_cast<double>(ax), static_cast<double>(ay),
+                  static_cast<double>(az));
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     const int mx = wave_max_i32(visits);
@@ -1171,28 +1185,34 @@ __global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__
     }
 }
 
-// The wave walk with each group of 64 particles split over two waves of one
-// workgroup.  S = the preorder slot of the group's middle particle's own
-// leaf; wave 0 owns the slots [0, S), wave 1 owns [S, 2n-1).  A node is
-// tested, counted and (if accepted) added only by its owning wave.  Wave 1
-// also tests the ancestors of slot S (slot < S, subtree reaching past S):
-// accepting one means wave 0 adds its monopole, so wave 1 skips the
-// subtree; subtrees ending at or before S are stepped over untested.  The
-// visited set per particle is the one-wave walk's; the two FP32 partial
-// sums are added through LDS.  Twice the waves, each half the walk.
+// The wave walk with each group of 64 particles split over SPLIT waves of
+// one workgroup (SPLIT = 2 or 4).  The boundaries are the preorder slots of
+// the group's own leaves at lanes 64 (SPLIT - j) / SPLIT, j = 1 .. SPLIT-1:
+// a right-first preorder visits leaves right to left, so these slots
+// increase with j.  Wave j owns the slots [b_j, b_j+1) (b_0 = 0, b_SPLIT =
+// 2n-1).  A node is tested, counted and (if accepted) added only by its
+// owning wave.  A wave also tests the ancestors of its first slot (slot <
+// b_j, subtree reaching past b_j): accepting one means an earlier wave adds
+// its monopole, so this wave skips the subtree; subtrees ending at or
+// before b_j are stepped over untested, since the walk resumes after a
+// subtree at its skip slot whether the subtree was accepted or opened.  The
+// visited set per particle is the one-wave walk's; the SPLIT FP32 partial
+// sums are added through LDS in wave order.
+template <int SPLIT>
 __global__ __launch_bounds__(256) void k_walk32_wave_split(
     const float4 *__restrict__ recA, const int2 *__restrict__ recB, const int *__restrict__ pre,
-    const int *__restrict__ order, double *__restrict__ acc, int n, float eps2,
-    unsigned long long *__restrict__ stats, int add)
+    const int *__restrict__ order, AccOut out, int n, float eps2,
+    unsigned long long *__restrict__ stats)
 {
-    __shared__ float s_part[2][64][3];  // wave 1's partial sums, per group
-    __shared__ int s_vis[2][64];
+    constexpr int G = 4 / SPLIT;  // groups of 64 particles per 256-thread block
+    __shared__ float s_part[G][SPLIT - 1][64][3];  // waves 1.. partial sums
+    __shared__ int s_vis[G][SPLIT - 1][64];
     const int wave = threadIdx.x >> 6;  // 0..3
-    const int grp = wave >> 1;          // group within the block
-    const int half = wave & 1;
+    const int grp = wave / SPLIT;       // group within the block
+    const int part = wave % SPLIT;
     const int lane = threadIdx.x & 63;
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int t = (b * 2 + grp) * 64 + lane;
+    const int t = (b * G + grp) * 64 + lane;
     const bool active = t < n;
     const int m = 2 * n - 1;
     int ps = 0;
@@ -1204,11 +1224,17 @@ __global__ __launch_bounds__(256) void k_walk32_wave_split(
         py = self.y;
         pz = self.z;
     }
-    // split slot: the middle particle's own leaf (lane 32, or the last active)
+    // the boundary slots (the last active lane stands in for inactive ones)
     const int nact = max(0, min(64, n - (t - lane)));
-    const int S = nact > 0 ? __builtin_amdgcn_readlane(ps, min(32, nact - 1)) : m;
-    const int lo = half ? S : 0;
-    const int hi = half ? m : S;
+    int lo = 0, hi = m;
+    if (nact > 0) {
+        if (part > 0)
+            lo = __builtin_amdgcn_readlane(ps, min(64 - 64 * part / SPLIT, nact - 1));
+        if (part < SPLIT - 1)
+            hi = __builtin_amdgcn_readlane(ps, min(64 - 64 * (part + 1) / SPLIT, nact - 1));
+    } else {
+        lo = hi = m;
+    }
     float ax = 0.f, ay = 0.f, az = 0.f;
     int visits = 0;
     int next = (active && lo < hi) ? 0 : m;
@@ -1262,29 +1288,205 @@ __global__ __launch_bounds__(256) void k_walk32_wave_split(
         else
             k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
     }
-    if (half == 1) {
-        s_part[grp][lane][0] = ax;
-        s_part[grp][lane][1] = ay;
-        s_part[grp][lane][2] = az;
-        s_vis[grp][lane] = visits;
+    if (part > 0) {
+        s_part[grp][part - 1][lane][0] = ax;
+        s_part[grp][part - 1][lane][1] = ay;
+        s_part[grp][part - 1][lane][2] = az;
+        s_vis[grp][part - 1][lane] = visits;
     }
     __syncthreads();
     int pv = 0;
-    if (half == 0) {
-        pv = visits + s_vis[grp][lane];
+    if (part == 0) {
+        pv = visits;
+#pragma unroll
+        for (int j = 0; j < SPLIT - 1; ++j) {
+            ax += s_part[grp][j][lane][0];
+            ay += s_part[grp][j][lane][1];
+            az += s_part[grp][j][lane][2];
+            pv += s_vis[grp][j][lane];
+        }
         if (active) {
             const int p = order[t];
-            store_acc(acc, p, static_cast<double>(ax + s_part[grp][lane][0]),
-                      static_cast<double>(ay + s_part[grp][lane][1]),
-                      static_cast<double>(az + s_part[grp][lane][2]), add);
+            store_acc(out, p, static_cast<double>(ax), static_cast<double>(ay),
+                      static_cast<double>(az));
         }
     }
     const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(visits));
     const int mx = wave_max_i32(pv);
     if (lane == 0) {
         atomicAdd(&stats[0], tot);
-        if (half == 0) atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+        if (part == 0) atomicMax(&stats[1], static_cast<unsigned long long>(mx));
     }
+}
+
+// Block b -> group, for launches of one block per group: the blocks go
+// round-robin over the 8 XCDs, and each XCD gets runs of R consecutive
+// (Morton-adjacent) groups, every 8th run, so the dense parts of a
+// distribution spread over all XCDs while neighbouring groups still share
+// an L2.  Blocks past the last full set of 8 runs map to themselves.
+__device__ __forceinline__ int xcd_run_block(int b, int nb, int R)
+{
+    const int span = 8 * R;
+    const int full = (nb / span) * span;
+    if (b >= full) return b;
+    const int x = b % 8, k = b / 8;
+    return ((k / R) * 8 + x) * R + k % R;
+}
+
+constexpr int kChunkMaxB = 30;  // chunk boundaries per group (13 per side + 1 inside + ends)
+
+// The wave walk of one group of 64 particles, cut into chunks of preorder
+// slots that the block's waves take in turn.  Most of a group's walk is in
+// its far field: the subtrees of leaves 2^10 .. 2^17 positions away in
+// Morton order.  The boundaries are the slots of the leaves 64 * 4^i
+// positions beyond each end of the group (on both sides) and of its own
+// leaf at lane 32; a right-first preorder visits leaves right to left, so
+// these slots increase from the far right through the group to the far
+// left.  A chunk [lo, hi) is walked like one part of the split walk
+// (ancestors of lo tested, earlier subtrees stepped over).  Waves take the
+// chunks farthest-first, alternating ends, from an LDS counter; each
+// chunk's FP32 partial sums go to LDS and are added in slot order at the
+// end, so the result does not depend on which wave walked which chunk.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
+    const float4 *__restrict__ recA, const int2 *__restrict__ recB, const int *__restrict__ pre,
+    const int *__restrict__ order, AccOut out, int n, float eps2,
+    unsigned long long *__restrict__ stats)
+{
+    __shared__ int s_b[kChunkMaxB + 1];
+    __shared__ int s_nc;
+    __shared__ int s_ctr;
+    __shared__ float s_part[kChunkMaxB][3][64];
+    __shared__ int s_vis[64];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int g = xcd_run_block(blockIdx.x, gridDim.x, 8);
+    const int t0 = g * 64;
+    const int t = t0 + lane;
+    const bool active = t < n;
+    const int m = 2 * n - 1;
+    const int nact = min(64, n - t0);
+    const int last = t0 + nact - 1;
+    // boundary leaves: right side at distances 64 * 4^i (farthest first),
+    // then lane 32, then the left side (nearest first)
+    int nr = 0, nl = 0;
+    for (long long d = 64; last + d <= n - 1; d *= 4) ++nr;
+    for (long long d = 64; t0 - d >= 0; d *= 4) ++nl;
+    const int ni = nact > 33 ? 1 : 0;
+    const int nbd = nr + ni + nl;  // interior boundaries (<= 13 + 1 + 13)
+    if (wave == 0) {
+        if (lane < nbd) {
+            int leaf;
+            if (lane < nr)
+                leaf = last + (64 << (2 * (nr - 1 - lane)));
+            else if (lane < nr + ni)
+                leaf = t0 + 32;
+            else
+                leaf = t0 - (64 << (2 * (lane - nr - ni)));
+            s_b[lane + 1] = pre[leaf];
+        }
+        if (lane == 0) {
+            s_b[0] = 0;
+            s_b[nbd + 1] = m;
+            s_nc = nbd + 1;
+            s_ctr = 0;
+        }
+        s_vis[lane] = 0;
+    }
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (active) {
+        const float4 self = recA[pre[t]];
+        px = self.x;
+        py = self.y;
+        pz = self.z;
+    }
+    __syncthreads();
+    const int nc = s_nc;
+    int wvis = 0;
+    while (true) {
+        int j = 0;
+        if (lane == 0) j = atomicAdd(&s_ctr, 1);
+        j = __builtin_amdgcn_readfirstlane(__shfl(j, 0, 64));
+        if (j >= nc) break;
+        const int c = (j & 1) ? nc - 1 - (j >> 1) : (j >> 1);
+        const int lo = s_b[c], hi = s_b[c + 1];
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        int visits = 0;
+        int next = (active && lo < hi) ? 0 : m;
+        int k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+        int k0 = -64;
+        float4 wa = make_float4(0.f, 0.f, 0.f, 0.f);
+        int2 wb = make_int2(0, 0);
+        while (k < hi) {
+            if (k - k0 >= 64) {
+                k0 = k;
+                const int slot = k0 + lane;
+                if (slot < m) {
+                    wa = recA[slot];
+                    wb = recB[slot];
+                }
+            }
+            const int l = k - k0;
+            const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
+            const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
+            const float cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
+            const float cw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
+            const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+            const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
+            if (next == k) {
+                if (skip <= lo) {
+                    next = skip;  // subtree entirely before this chunk
+                } else {
+                    const bool own = k >= lo;
+                    const float dx = cx - px;
+                    const float dy = cy - py;
+                    const float dz = cz - pz;
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+                    if (crit < r2) {
+                        if (own) {
+                            ++visits;
+                            const float inv = __builtin_amdgcn_rsqf(r2);
+                            const float s = cw * (inv * inv) * inv;
+                            ax = fmaf(s, dx, ax);
+                            ay = fmaf(s, dy, ay);
+                            az = fmaf(s, dz, az);
+                        }
+                        next = skip;
+                    } else {
+                        visits += own ? 1 : 0;
+                        next = k + 1;
+                    }
+                }
+            }
+            if (__ballot(next == k + 1))
+                k = k + 1;
+            else
+                k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+        }
+        s_part[c][0][lane] = ax;
+        s_part[c][1][lane] = ay;
+        s_part[c][2][lane] = az;
+        if (visits) atomicAdd(&s_vis[lane], visits);
+        wvis += visits;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        for (int c = 0; c < nc; ++c) {
+            ax += s_part[c][0][lane];
+            ay += s_part[c][1][lane];
+            az += s_part[c][2][lane];
+        }
+        if (active) {
+            const int p = order[t];
+            store_acc(out, p, static_cast<double>(ax), static_cast<double>(ay),
+                      static_cast<double>(az));
+        }
+        const int mx = wave_max_i32(s_vis[lane]);
+        if (lane == 0) atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+    }
+    const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(wvis));
+    if (lane == 0) atomicAdd(&stats[0], tot);
 }
 
 }  // namespace
@@ -1357,7 +1559,7 @@ int bh_mark(rg_bh *h, int phase, hipStream_t st)
 
 int rg::rg_bh_set_walk_mode_impl(int64_t v)
 {
-    RG_REQUIRE(v >= -1 && v <= 4, RG_ERR_ARG, "walk_mode must be -1 (auto) or 0..4");
+    RG_REQUIRE(v >= -1 && v <= 6, RG_ERR_ARG, "walk_mode must be -1 (auto) or 0..6");
     g_walk_mode = static_cast<int>(v);
     return RG_OK;
 }
@@ -1447,15 +1649,19 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
 
 namespace {
 
-int bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc, int add,
+int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &out,
                void *stream)
 {
     RG_REQUIRE(h, RG_ERR_ARG, "null handle");
-    RG_REQUIRE(pos && mass && acc, RG_ERR_ARG, "null pointer argument");
+    RG_REQUIRE(pos && mass && out.acc, RG_ERR_ARG, "null pointer argument");
     const int n = static_cast<int>(h->n);
     const size_t b3 = 24 * static_cast<size_t>(n), b1 = 8 * static_cast<size_t>(n);
-    RG_REQUIRE(!rg_overlap(acc, b3, pos, b3) && !rg_overlap(acc, b3, mass, b1), RG_ERR_ALIAS,
-               "acc aliases pos or mass");
+    RG_REQUIRE(!rg_overlap(out.acc, b3, pos, b3) && !rg_overlap(out.acc, b3, mass, b1),
+               RG_ERR_ALIAS, "acc aliases pos or mass");
+    if (out.vel)
+        RG_REQUIRE(!rg_overlap(out.vel, b3, pos, b3) && !rg_overlap(out.vel, b3, mass, b1) &&
+                       !rg_overlap(out.vel, b3, out.acc, b3),
+                   RG_ERR_ALIAS, "vel aliases pos, mass or acc");
     int dev = 0;
     RG_HIP(hipGetDevice(&dev));
     RG_REQUIRE(dev == h->device, RG_ERR_ARG, "handle was created on device %d, current is %d",
@@ -1547,33 +1753,42 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc, int
     const double eps2 = h->eps * h->eps;
     h->walk_used = -1;
     if (fp64) {
-        k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, acc, n, h->G, theta2, eps2, h->stats,
-                                    add);
+        k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, out, n, h->G, theta2, eps2, h->stats);
     } else {
-        // Below ~400k particles the walk has too few waves to hide its step
-        // latency, and splitting each group over two waves pays; above, the
-        // split's extra ancestor tests cost more than it hides.
-        const int mode = g_walk_mode >= 0 ? g_walk_mode : (n < 400000 ? 4 : 2);
+        // The chunked walk measured fastest from 60k to 1M particles: its
+        // time is set by the longest wave, and chunking the far field
+        // shortens that wave the most.
+        const int mode = g_walk_mode >= 0 ? g_walk_mode : 6;
         h->walk_used = mode;
-        if (mode == 4) {
-            k_walk32_wave_split<<<(n + 127) / 128, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
-                                                                 acc, n, static_cast<float>(eps2),
-                                                                 h->stats, add);
+        if (mode == 6) {
+            k_walk32_chunked<4><<<(n + 63) / 64, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
+                                                               out, n, static_cast<float>(eps2),
+                                                               h->stats);
+        } else if (mode == 5) {
+            k_walk32_wave_split<4><<<(n + 63) / 64, 256, 0, st>>>(h->recA, h->recB, h->pre,
+                                                                  order, out, n,
+                                                                  static_cast<float>(eps2),
+                                                                  h->stats);
+        } else if (mode == 4) {
+            k_walk32_wave_split<2><<<(n + 127) / 128, 256, 0, st>>>(h->recA, h->recB, h->pre,
+                                                                   order, out, n,
+                                                                   static_cast<float>(eps2),
+                                                                   h->stats);
         } else if (mode == 3) {
             k_walk32_wave2<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
-                                                            acc, n, static_cast<float>(eps2),
-                                                            h->stats, add);
+                                                            out, n, static_cast<float>(eps2),
+                                                            h->stats);
         } else if (mode == 2) {
             k_walk32_wave<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
-                                                           acc, n, static_cast<float>(eps2),
-                                                           h->stats, add);
+                                                           out, n, static_cast<float>(eps2),
+                                                           h->stats);
         } else if (mode == 1) {
             const int wg2 = (2 * n + wb - 1) / wb;
-            k_walk32_pair<<<wg2, wb, 0, st>>>(h->recA, h->recB, h->pre, order, acc, n,
-                                              static_cast<float>(eps2), h->stats, add);
+            k_walk32_pair<<<wg2, wb, 0, st>>>(h->recA, h->recB, h->pre, order, out, n,
+                                              static_cast<float>(eps2), h->stats);
         } else {
-            k_walk32<<<wg, wb, 0, st>>>(h->recA, h->recB, h->pre, order, acc, n,
-                                        static_cast<float>(eps2), h->stats, add);
+            k_walk32<<<wg, wb, 0, st>>>(h->recA, h->recB, h->pre, order, out, n,
+                                        static_cast<float>(eps2), h->stats);
         }
     }
     RG_CHECK_LAUNCH();
@@ -1590,13 +1805,20 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc, int
 extern "C" int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc,
                              void *stream)
 {
-    return bh_compute(h, pos, mass, acc, 0, stream);
+    return bh_compute(h, pos, mass, AccOut{acc, nullptr, 0.0, 0}, stream);
 }
 
 extern "C" int rg_bh_compute_add(rg_bh *h, const double *pos, const double *mass, double *acc,
                                  void *stream)
 {
-    return bh_compute(h, pos, mass, acc, 1, stream);
+    return bh_compute(h, pos, mass, AccOut{acc, nullptr, 0.0, 1}, stream);
+}
+
+extern "C" int rg_bh_compute_kick(rg_bh *h, const double *pos, const double *mass, double *acc,
+                                  double *vel, double dt_half, void *stream)
+{
+    RG_REQUIRE(vel, RG_ERR_ARG, "null vel");
+    return bh_compute(h, pos, mass, AccOut{acc, vel, dt_half, 0}, stream);
 }
 
 extern "C" int rg_bh_set_timing(rg_bh *h, int enable)

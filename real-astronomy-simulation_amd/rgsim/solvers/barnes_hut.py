@@ -66,6 +66,16 @@ class BarnesHut:
                                              acc.data_ptr(), _lib.stream_ptr(pos)),
                    "rg_bh_compute")
 
+    def compute_kick(self, pos, mass, acc, vel, dt_half) -> None:
+        """compute(pos, mass, acc) then vel += acc * dt_half (leapfrog.py:11-15),
+        the kick fused into the walk: the closing half of a KDK step."""
+        n = self.n
+        vec3(pos, n, "pos"), vec1(mass, n, "mass"), vec3(acc, n, "acc"), vec3(vel, n, "vel")
+        _lib.check(_lib.load().rg_bh_compute_kick(self._h, pos.data_ptr(), mass.data_ptr(),
+                                                  acc.data_ptr(), vel.data_ptr(),
+                                                  float(dt_half), _lib.stream_ptr(pos)),
+                   "rg_bh_compute_kick")
+
     def compute_add(self, pos, mass, acc) -> None:
         """acc += the treecode acceleration: compute(pos, mass, acc_g) then
         acc += acc_g, as living_galaxy_engine.py:1017-1018 and

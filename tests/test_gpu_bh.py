@@ -182,7 +182,7 @@ def test_fp32_walk_modes_agree(name):
     from rgsim import _lib
     pos, mass = CASES[name]
     res = {}
-    for mode in (0, 1, 2, 3, 4):
+    for mode in (0, 1, 2, 3, 4, 5, 6):
         _lib.set_param("walk_mode", mode)
         try:
             bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
@@ -190,17 +190,17 @@ def test_fp32_walk_modes_agree(name):
         finally:
             _lib.set_param("walk_mode", -1)
     (a0, s0) = res[0]
-    for mode in (1, 2, 3, 4):
+    for mode in (1, 2, 3, 4, 5, 6):
         s = res[mode][1]
         assert s["walk_mode"] == mode
         assert s0["visits_total"] == s["visits_total"] and s0["visits_max"] == s["visits_max"], mode
-    # the automatic choice below 400k particles is the split wave walk
+    # the automatic choice is the chunked wave walk
     bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
-    assert bh.stats()["walk_mode"] == 4 and np.array_equal(acc, res[4][0])
+    assert bh.stats()["walk_mode"] == 6 and np.array_equal(acc, res[6][0])
     bh, _ = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp64")
     assert bh.stats()["walk_mode"] == -1
     scale = np.linalg.norm(a0, axis=1).max()
-    for mode in (1, 4):  # two partial sums added at the end
+    for mode in (1, 4, 5, 6):  # partial sums (per lane pair, wave or chunk) added at the end
         assert np.abs(a0 - res[mode][0]).max() <= 1e-5 * scale, mode
     for mode in (2, 3):  # each lane's summation order unchanged
         assert np.array_equal(a0, res[mode][0]), mode
