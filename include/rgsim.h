@@ -60,6 +60,12 @@ const char *rg_last_error(void);
  *                 the wave walk split over two / four waves per particle
  *                 group, 6 the wave walk over far-field chunks of the
  *                 preorder that four waves per group take in turn
+ *   "chunk_dmin", "chunk_shift", "chunk_run", "chunk_waves"
+ *                 shape of walk 6: leaf distance of the nearest chunk
+ *                 boundary beyond each end of a group (default 256); growth
+ *                 factor 2^shift of the next ones (0 = by n: 1 below 700k
+ *                 particles, else 2); runs of groups per XCD (32); waves per
+ *                 group, 4 or 8 (0 = by n: 8 below 100k particles)
  * Process-wide; takes effect at the next launch. */
 int rg_set_param(const char *name, int64_t value);
 

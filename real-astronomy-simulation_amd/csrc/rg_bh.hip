@@ -12,12 +12,12 @@
 //                  permutation equals np.argsort(kind="stable") (:247)
 //   4. k_build     Karras build with the reference's exact delta (incl. the
 //                  index tie-break) using hardware clz (:50-121)
-//   5. k_agg_*     leaf init + deterministic bottom-up aggregation: the
-//                  second child to finish combines both children, so every
-//                  node is computed exactly once and bitwise reproducibly
-//                  (:124-177).  Three levels, each synchronised inside one
-//                  workgroup: treelets of <= 256 leaves in LDS, mid-treelets
-//                  of <= 65536 leaves, and the top tree above them.
+//   5. k_agg_*     leaf init + deterministic aggregation (:124-177), every
+//                  node computed exactly once and bitwise reproducibly:
+//                  treelets of <= 256 leaves bottom-up in LDS (the second
+//                  child to finish combines both), then every node above
+//                  them on its own as the sum of its run of treelets, in a
+//                  fixed order (k_treelet_rank orders the treelets).
 //   6. k_walk*     stackless walk in right-first DFS order: the node
 //                  sequence the reference's stack walk pops (:180-217).
 //                  FP64 follows right-child/escape links over node-id
@@ -53,6 +53,12 @@ int g_walk_bs = kWalkBSDefault;
 // slots per step, 4 / 5 wave-coherent split over two / four waves per particle
 // group, 6 wave-coherent over far-field chunks taken by four waves
 int g_walk_mode = -1;
+// chunked walk shape: nearest boundary distance, growth 2^shift (0: by n),
+// XCD run length, waves per group (0: by n)
+int g_chunk_dmin = 256;
+int g_chunk_shift = 0;
+int g_chunk_run = 32;
+int g_chunk_waves = 0;
 
 struct alignas(16) WalkRec64 {
     double x, y, z, m;   // centre of mass, mass
@@ -380,8 +386,8 @@ __device__ __forceinline__ int delta_dev(int64_t i, int64_t j, int64_t n, uint64
 __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restrict__ left,
                         int *__restrict__ right, int *__restrict__ parent,
                         int *__restrict__ up, int2 *__restrict__ range,
-                        int *__restrict__ left_at_split, int *__restrict__ arrive,
-                        int *__restrict__ counts, int *__restrict__ tl_of)
+                        int *__restrict__ left_at_split, int *__restrict__ counts,
+                        int *__restrict__ tl_of)
 {
     const int64_t nn = n;
     for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += gridDim.x * blockDim.x)
@@ -420,12 +426,12 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
         up[rc] = (self << 1) | 1;
         range[i] = make_int2(static_cast<int>(lo), static_cast<int>(hi));
         left_at_split[gamma] = lc;
-        arrive[i] = 0;
         if (i == 0) {
             parent[nn] = -1;
             up[nn] = -1;
             counts[0] = 0;  // treelets
-            counts[1] = 0;  // mid-treelets
+            counts[1] = 0;  // nodes above the treelets
+            counts[2] = 0;  // treelet-rank tile tickets
         }
     }
 }
@@ -517,29 +523,121 @@ __device__ __forceinline__ void write_node(Agg *agg, const WalkOut &wo,
     }
 }
 
-constexpr int kTreelet = 256;    // max leaves of a treelet (one workgroup, LDS)
-constexpr int kMidTree = 65536;  // max leaves of a mid-treelet (one workgroup, global)
+constexpr int kTreelet = 256;  // max leaves of a treelet (one workgroup, LDS)
+constexpr int kRankItems = 16;
+constexpr int kRankTile = 256 * kRankItems;  // leaves per treelet-rank tile
 
 // Treelets: the maximal subtrees with at most kTreelet leaves (a node is a
 // treelet root when its leaf count is <= kTreelet and its parent's is
-// larger); mid-treelets likewise with kMidTree.  Both partition the leaves.
-// tl_of[first leaf of a treelet] = its root (other leaves keep -1).
+// larger); they partition the leaves.  tl_of[first leaf of a treelet] = its
+// root (other leaves keep -1); the internal nodes above the treelets are
+// listed in uppers.  Also clears the treelet-rank look-back words.
 __global__ void k_classify(const int *__restrict__ parent, const int2 *__restrict__ range, int n,
-                           int *__restrict__ treelets, int *__restrict__ mids,
-                           int *__restrict__ counts, int *__restrict__ tl_of)
+                           int *__restrict__ treelets, int *__restrict__ uppers,
+                           int *__restrict__ counts, int *__restrict__ tl_of,
+                           unsigned *__restrict__ rank_lb, int rank_tiles)
 {
     const int m = 2 * n - 1;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m; v += gridDim.x * blockDim.x) {
+        if (v < rank_tiles) rank_lb[v] = 0u;
         const int lo = v < n ? v : range[v - n].x;
         const int size = v < n ? 1 : range[v - n].y - lo + 1;
-        if (size > kMidTree) continue;
+        if (size > kTreelet) {
+            uppers[atomicAdd(&counts[1], 1)] = v;
+            continue;
+        }
         const int p = parent[v];
         const int psize = p < 0 ? INT_MAX : range[p - n].y - range[p - n].x + 1;
-        if (size <= kTreelet && psize > kTreelet) {
+        if (psize > kTreelet) {
             treelets[atomicAdd(&counts[0], 1)] = v;
             tl_of[lo] = v;
         }
-        if (psize > kMidTree) mids[atomicAdd(&counts[1], 1)] = v;
+    }
+}
+
+// Rank of every treelet in leaf order: an exclusive scan over the leaves of
+// "first leaf of a treelet", one tile of kRankTile leaves per workgroup with
+// decoupled look-back (the sort's scheme: one word per tile, status + count,
+// bounded waits).  trank[first leaf] = rank, tsorted[rank] = root, and
+// counts[3] = the number of treelets.
+__global__ __launch_bounds__(256) void k_treelet_rank(const int *__restrict__ tl_of, int n,
+                                                      int *__restrict__ trank,
+                                                      int *__restrict__ tsorted,
+                                                      unsigned *__restrict__ lb,
+                                                      int *__restrict__ counts,
+                                                      unsigned long long *__restrict__ stats)
+{
+    __shared__ int s_tile;
+    __shared__ unsigned s_wsum[4];
+    __shared__ unsigned s_excl;
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    const int lane = t & 63;
+    if (t == 0) s_tile = atomicAdd(&counts[2], 1);
+    __syncthreads();
+    const int tile = s_tile;
+    const int base = tile * kRankTile + t * kRankItems;
+    unsigned bits = 0;
+#pragma unroll
+    for (int r = 0; r < kRankItems; ++r) {
+        const int i = base + r;
+        if (i < n && tl_of[i] >= 0) bits |= 1u << r;
+    }
+    const unsigned cnt = __popc(bits);
+    unsigned incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) s_wsum[w] = incl;
+    __syncthreads();
+    unsigned woff = 0, total = 0;
+#pragma unroll
+    for (int ww = 0; ww < 4; ++ww) {
+        if (ww < w) woff += s_wsum[ww];
+        total += s_wsum[ww];
+    }
+    if (t == 0) {
+        unsigned *mine = &lb[tile];
+        __hip_atomic_store(mine, (tile == 0 ? kFlagInc : kFlagAgg) | total, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        unsigned excl = 0;
+        if (tile > 0) {
+            int j = tile - 1;
+            int spins = 0;
+            while (j >= 0) {
+                const unsigned v = __hip_atomic_load(&lb[j], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned f = v & ~kCountMask;
+                if (f == 0u) {
+                    if (++spins > kSpinLimit) {
+                        atomicAdd(&stats[3], 1ull);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += v & kCountMask;
+                if (f == kFlagInc) break;
+                --j;
+            }
+            __hip_atomic_store(mine, kFlagInc | (excl + total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_excl = excl;
+        if (tile == static_cast<int>(gridDim.x) - 1) counts[3] = static_cast<int>(excl + total);
+    }
+    __syncthreads();
+    unsigned r0 = s_excl + woff + incl - cnt;
+#pragma unroll
+    for (int r = 0; r < kRankItems; ++r) {
+        if ((bits >> r) & 1u) {
+            const int i = base + r;
+            trank[i] = static_cast<int>(r0);
+            tsorted[r0] = tl_of[i];
+            ++r0;
+        }
     }
 }
 
@@ -629,114 +727,83 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     }
 }
 
-// A node's topology, loaded before its arrival counter so the loads overlap
-// the atomic's round trip.
-struct Topo {
-    int L, R, up;
-    int2 rg;
-};
-
-__device__ __forceinline__ Topo load_topo(int node, int n, const int *__restrict__ left,
-                                          const int *__restrict__ right,
-                                          const int *__restrict__ parent,
-                                          const int2 *__restrict__ range)
-{
-    const int ii = node - n;
-    return Topo{left[ii], right[ii], parent[node], range[ii]};
-}
-
-// Combine a node from its children's aggregates (global) and write it.
+// The nodes above the treelets: each is the sum of a run of consecutive
+// treelets in leaf order, [trank[lo], trank[hi + 1]), so every upper node is
+// summed on its own, with no ordering between nodes -- one workgroup per
+// node.  Thread t adds treelets A + t, A + t + 256, ... in order; the 256
+// partials are combined by a butterfly within each wave (both operands of
+// every add are the same on the two lanes, so all lanes agree) and the four
+// wave results in wave order: a fixed summation order.  Min/max are exact.
 template <int MODE>
-__device__ __forceinline__ void combine_global(int node, int n, const Topo &tp,
-                                               const int *__restrict__ left_at_split,
-                                               Agg *agg, const WalkOut &wo)
+__global__ __launch_bounds__(256) void k_agg_upper(
+    int n, const int *__restrict__ right, const int2 *__restrict__ range,
+    const int *__restrict__ left_at_split, const int *__restrict__ uppers,
+    const int *__restrict__ counts, const int *__restrict__ trank,
+    const int *__restrict__ tsorted, Agg *agg, WalkOut wo)
 {
-    const Agg aL = agg[tp.L];
-    const Agg aR = agg[tp.R];
-    const double mm = aL.m + aR.m;
-    double mom[3], mn[3], mx[3], c[3];
+    __shared__ double s_r[4][10];
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    const int lane = t & 63;
+    const int nup = counts[1];
+    const int T = counts[3];
+    for (int e = blockIdx.x; e < nup; e += gridDim.x) {
+        const int v = uppers[e];
+        const int ii = v - n;
+        const int2 rg = range[ii];
+        const int A = trank[rg.x];
+        const int B = rg.y == n - 1 ? T : trank[rg.y + 1];
+        double r[10] = {0.0, 0.0, 0.0, 0.0, INFINITY, INFINITY, INFINITY,
+                        -INFINITY, -INFINITY, -INFINITY};
+        for (int j = A + t; j < B; j += 256) {
+            const Agg a = agg[tsorted[j]];
+            r[0] += a.m;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        mom[a] = aL.mom[a] + aR.mom[a];
-        mn[a] = fmin(aL.mn[a], aR.mn[a]);
-        mx[a] = fmax(aL.mx[a], aR.mx[a]);
-        c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
-    }
-    write_node<MODE>(agg, wo, node, n, mm, mom, mn, mx, c, tp.rg.x, tp.rg.y, tp.R,
-                     tp.rg.x == 0 ? -1 : left_at_split[tp.rg.x - 1]);
-}
-
-// Pass 2: one workgroup per mid-treelet.  Its threads find the treelet
-// roots inside it (tl_of on the first leaves) and climb from each; both
-// children of every node they meet lie inside the mid-treelet, so the
-// arrival counters (global) and the node data are synchronised at
-// workgroup scope (acq_rel).  The climb stops at the mid-treelet root.
-constexpr int kMidBS = 1024;
-
-template <int MODE>
-__global__ __launch_bounds__(kMidBS) void k_agg_mid(
-    int n, const int *__restrict__ left, const int *__restrict__ right,
-    const int *__restrict__ parent, const int2 *__restrict__ range,
-    const int *__restrict__ left_at_split, int *__restrict__ arrive,
-    const int *__restrict__ mids, const int *__restrict__ counts,
-    const int *__restrict__ tl_of, int *__restrict__ entries, Agg *agg, WalkOut wo)
-{
-    __shared__ int s_cnt;
-    const int cnt = counts[1];
-    for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
-        const int M = mids[e];  // uniform
-        if (M < n) continue;    // a single leaf
-        const int2 mr = range[M - n];
-        // Collect the treelet roots inside M (the scan's loads are independent)
-        // into entries[mr.x ...] -- M's own leaf range, so no overflow and no
-        // overlap with another mid-treelet -- then climb from all of them.
-        if (threadIdx.x == 0) s_cnt = 0;
-        __syncthreads();
-#pragma unroll 4
-        for (int li = mr.x + static_cast<int>(threadIdx.x); li <= mr.y; li += kMidBS) {
-            const int r = tl_of[li];
-            if (r >= 0 && r != M) entries[mr.x + atomicAdd(&s_cnt, 1)] = r;
-        }
-        __syncthreads();  // workgroup-scope visibility of entries[] (global) and s_cnt
-        const int ne = s_cnt;
-        for (int k = threadIdx.x; k < ne; k += kMidBS) {
-            int node = parent[entries[mr.x + k]];
-            while (true) {
-                const Topo tp = load_topo(node, n, left, right, parent, range);
-                if (__hip_atomic_fetch_add(&arrive[node - n], 1, __ATOMIC_ACQ_REL,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-                    break;
-                combine_global<MODE>(node, n, tp, left_at_split, agg, wo);
-                if (node == M) break;
-                node = tp.up;
+            for (int c = 0; c < 3; ++c) {
+                r[1 + c] += a.mom[c];
+                r[4 + c] = fmin(r[4 + c], a.mn[c]);
+                r[7 + c] = fmax(r[7 + c], a.mx[c]);
             }
         }
-        __syncthreads();  // s_cnt is reused by the next mid-treelet
-    }
-}
-
-// Pass 3: one workgroup climbs from the mid-treelet roots to the root.
-constexpr int kTopBS = 1024;
-
-template <int MODE>
-__global__ __launch_bounds__(kTopBS) void k_agg_top(
-    int n, const int *__restrict__ left, const int *__restrict__ right,
-    const int *__restrict__ parent, const int2 *__restrict__ range,
-    const int *__restrict__ left_at_split, int *__restrict__ arrive,
-    const int *__restrict__ mids, const int *__restrict__ counts, Agg *agg,
-    WalkOut wo)
-{
-    const int cnt = counts[1];
-    for (int e = threadIdx.x; e < cnt; e += kTopBS) {
-        int node = parent[mids[e]];
-        while (node >= 0) {
-            const Topo tp = load_topo(node, n, left, right, parent, range);
-            if (__hip_atomic_fetch_add(&arrive[node - n], 1, __ATOMIC_ACQ_REL,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-                break;
-            combine_global<MODE>(node, n, tp, left_at_split, agg, wo);
-            node = tp.up;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) r[c] += __shfl_xor(r[c], off, 64);
+#pragma unroll
+            for (int c = 4; c < 7; ++c) r[c] = fmin(r[c], __shfl_xor(r[c], off, 64));
+#pragma unroll
+            for (int c = 7; c < 10; ++c) r[c] = fmax(r[c], __shfl_xor(r[c], off, 64));
         }
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < 10; ++c) s_r[w][c] = r[c];
+        __syncthreads();
+        if (t == 0) {
+            double q[10];
+#pragma unroll
+            for (int c = 0; c < 10; ++c) q[c] = s_r[0][c];
+            for (int ww = 1; ww < 4; ++ww) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) q[c] += s_r[ww][c];
+#pragma unroll
+                for (int c = 4; c < 7; ++c) q[c] = fmin(q[c], s_r[ww][c]);
+#pragma unroll
+                for (int c = 7; c < 10; ++c) q[c] = fmax(q[c], s_r[ww][c]);
+            }
+            const double mm = q[0];
+            double mom[3], mn[3], mx[3], cm[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                mom[c] = q[1 + c];
+                mn[c] = q[4 + c];
+                mx[c] = q[7 + c];
+                // barnes_hut.py:175-177: com = mom / mass where mass > 0.
+                cm[c] = mm > 0.0 ? mom[c] / mm : 0.0;
+            }
+            write_node<MODE>(agg, wo, v, n, mm, mom, mn, mx, cm, rg.x, rg.y, right[ii],
+                             rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
+        }
+        __syncthreads();  // s_r is reused by the next node
     }
 }
 
@@ -1333,7 +1400,8 @@ __device__ __forceinline__ int xcd_run_block(int b, int nb, int R)
     return ((k / R) * 8 + x) * R + k % R;
 }
 
-constexpr int kChunkMaxB = 30;  // chunk boundaries per group (13 per side + 1 inside + ends)
+constexpr int kChunkSide = 14;                     // chunk boundaries per side, at most
+constexpr int kChunkMaxB = 2 * kChunkSide + 2;    // + 1 inside the group + 1 end
 
 // The wave walk of one group of 64 particles, cut into chunks of preorder
 // slots that the block's waves take in turn.  Most of a group's walk is in
@@ -1347,11 +1415,17 @@ constexpr int kChunkMaxB = 30;  // chunk boundaries per group (13 per side + 1 i
 // chunks farthest-first, alternating ends, from an LDS counter; each
 // chunk's FP32 partial sums go to LDS and are added in slot order at the
 // end, so the result does not depend on which wave walked which chunk.
+struct ChunkShape {
+    int dmin;   // distance (in leaves) of the nearest boundary beyond each end
+    int shift;  // boundary distances grow by 2^shift
+    int run;    // XCD run length (groups)
+};
+
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     const float4 *__restrict__ recA, const int2 *__restrict__ recB, const int *__restrict__ pre,
     const int *__restrict__ order, AccOut out, int n, float eps2,
-    unsigned long long *__restrict__ stats)
+    unsigned long long *__restrict__ stats, ChunkShape cs)
 {
     __shared__ int s_b[kChunkMaxB + 1];
     __shared__ int s_nc;
@@ -1360,29 +1434,30 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     __shared__ int s_vis[64];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int g = xcd_run_block(blockIdx.x, gridDim.x, 8);
+    const int g = xcd_run_block(blockIdx.x, gridDim.x, cs.run);
     const int t0 = g * 64;
     const int t = t0 + lane;
     const bool active = t < n;
     const int m = 2 * n - 1;
     const int nact = min(64, n - t0);
     const int last = t0 + nact - 1;
-    // boundary leaves: right side at distances 64 * 4^i (farthest first),
-    // then lane 32, then the left side (nearest first)
+    // boundary leaves: right side at distances dmin * 2^(shift i) (farthest
+    // first), then lane 32, then the left side (nearest first); at most
+    // kChunkSide per side, the nearest ones (the farthest chunk takes the rest)
     int nr = 0, nl = 0;
-    for (long long d = 64; last + d <= n - 1; d *= 4) ++nr;
-    for (long long d = 64; t0 - d >= 0; d *= 4) ++nl;
+    for (long long d = cs.dmin; last + d <= n - 1 && nr < kChunkSide; d <<= cs.shift) ++nr;
+    for (long long d = cs.dmin; t0 - d >= 0 && nl < kChunkSide; d <<= cs.shift) ++nl;
     const int ni = nact > 33 ? 1 : 0;
-    const int nbd = nr + ni + nl;  // interior boundaries (<= 13 + 1 + 13)
+    const int nbd = nr + ni + nl;  // interior boundaries (<= 2 kChunkSide + 1)
     if (wave == 0) {
         if (lane < nbd) {
-            int leaf;
+            long long leaf;
             if (lane < nr)
-                leaf = last + (64 << (2 * (nr - 1 - lane)));
+                leaf = last + (static_cast<long long>(cs.dmin) << (cs.shift * (nr - 1 - lane)));
             else if (lane < nr + ni)
                 leaf = t0 + 32;
             else
-                leaf = t0 - (64 << (2 * (lane - nr - ni)));
+                leaf = t0 - (static_cast<long long>(cs.dmin) << (cs.shift * (lane - nr - ni)));
             s_b[lane + 1] = pre[leaf];
         }
         if (lane == 0) {
@@ -1512,9 +1587,11 @@ struct rg_bh {
     int *tickets = nullptr;
     int *left = nullptr, *right = nullptr, *parent = nullptr;
     int2 *range = nullptr;
-    int *left_at_split = nullptr, *arrive = nullptr;
-    int *up = nullptr, *pre = nullptr, *treelets = nullptr, *mids = nullptr;
-    int *counts = nullptr, *tl_of = nullptr, *entries = nullptr;
+    int *left_at_split = nullptr;
+    int *up = nullptr, *pre = nullptr, *treelets = nullptr, *uppers = nullptr;
+    int *counts = nullptr, *tl_of = nullptr, *trank = nullptr, *tsorted = nullptr;
+    unsigned *rank_lb = nullptr;  // treelet-rank look-back words, one per tile
+    int rank_tiles = 0;
     Agg *agg = nullptr;
     WalkRec64 *rec64 = nullptr;  // FP64 walk records (node-id order)
     float4 *recA = nullptr;      // FP32 walk records (preorder)
@@ -1561,6 +1638,27 @@ int rg::rg_bh_set_walk_mode_impl(int64_t v)
 {
     RG_REQUIRE(v >= -1 && v <= 6, RG_ERR_ARG, "walk_mode must be -1 (auto) or 0..6");
     g_walk_mode = static_cast<int>(v);
+    return RG_OK;
+}
+
+int rg::rg_bh_set_chunk_impl(const char *what, int64_t v)
+{
+    if (std::strcmp(what, "dmin") == 0) {
+        RG_REQUIRE(v >= 1 && v <= (1 << 20), RG_ERR_ARG, "chunk_dmin must be in [1, 2^20]");
+        g_chunk_dmin = static_cast<int>(v);
+    } else if (std::strcmp(what, "shift") == 0) {
+        RG_REQUIRE(v >= 0 && v <= 4, RG_ERR_ARG, "chunk_shift must be 0 (auto) or 1..4");
+        g_chunk_shift = static_cast<int>(v);
+    } else if (std::strcmp(what, "run") == 0) {
+        RG_REQUIRE(v >= 1 && v <= 4096, RG_ERR_ARG, "chunk_run must be in [1, 4096]");
+        g_chunk_run = static_cast<int>(v);
+    } else if (std::strcmp(what, "waves") == 0) {
+        RG_REQUIRE(v == 0 || v == 4 || v == 8, RG_ERR_ARG, "chunk_waves must be 0 (auto), 4 or 8");
+        g_chunk_waves = static_cast<int>(v);
+    } else {
+        rg::set_error("unknown parameter 'chunk_%s'", what);
+        return RG_ERR_ARG;
+    }
     return RG_OK;
 }
 
@@ -1624,14 +1722,16 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->right, 4 * (N - 1));
     ALLOC(h->range, 8 * (N - 1));
     ALLOC(h->left_at_split, 4 * (N - 1));
-    ALLOC(h->arrive, 4 * (N - 1));
     ALLOC(h->parent, 4 * M);
     ALLOC(h->up, 4 * M);
-    ALLOC(h->treelets, 4 * M);
-    ALLOC(h->mids, 4 * M);
+    ALLOC(h->treelets, 4 * N);
+    ALLOC(h->uppers, 4 * N);
     ALLOC(h->counts, 16);
     ALLOC(h->tl_of, 4 * N);
-    ALLOC(h->entries, 4 * N);
+    ALLOC(h->trank, 4 * N);
+    ALLOC(h->tsorted, 4 * N);
+    h->rank_tiles = static_cast<int>((N + kRankTile - 1) / kRankTile);
+    ALLOC(h->rank_lb, 4 * static_cast<size_t>(h->rank_tiles));
     ALLOC(h->agg, sizeof(Agg) * M);
     if (precision == RG_FP64) {
         ALLOC(h->rec64, sizeof(WalkRec64) * M);
@@ -1696,13 +1796,16 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
 
     const int g1 = min((n + 255) / 256, 16384);
     k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->up, h->range,
-                                h->left_at_split, h->arrive, h->counts, h->tl_of);
+                                h->left_at_split, h->counts, h->tl_of);
     RG_CHECK_LAUNCH();
     const bool fp64 = h->precision == RG_FP64;
     {
         const int gc = min((2 * n - 1 + 255) / 256, 16384);
-        k_classify<<<gc, 256, 0, st>>>(h->parent, h->range, n, h->treelets, h->mids, h->counts,
-                                       h->tl_of);
+        k_classify<<<gc, 256, 0, st>>>(h->parent, h->range, n, h->treelets, h->uppers,
+                                       h->counts, h->tl_of, h->rank_lb, h->rank_tiles);
+        RG_CHECK_LAUNCH();
+        k_treelet_rank<<<h->rank_tiles, 256, 0, st>>>(h->tl_of, n, h->trank, h->tsorted,
+                                                      h->rank_lb, h->counts, h->stats);
         RG_CHECK_LAUNCH();
     }
     if (!fp64) {
@@ -1723,7 +1826,7 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     rc = bh_mark(h, 2, st);
     if (rc != RG_OK) return rc;
     const int gt = min(n / 64 + 64, 4096);
-    const int gm = min(n / 16384 + 8, 1024);
+    const int gu = min(n / 128 + 8, 8192);
 #define RG_AGG_LAUNCH(MODE)                                                                  \
     do {                                                                                     \
         k_agg_treelet<MODE><<<gt, kTreelet, 0, st>>>(order, pos, mass, n, h->left, h->right, \
@@ -1731,13 +1834,9 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
                                                     h->treelets, h->counts, h->agg,   \
                                                     wo);                                     \
         RG_CHECK_LAUNCH();                                                                   \
-        k_agg_mid<MODE><<<gm, kMidBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,    \
-                                               h->left_at_split, h->arrive, h->mids,         \
-                                               h->counts, h->tl_of, h->entries, h->agg, wo);      \
-        RG_CHECK_LAUNCH();                                                                   \
-        k_agg_top<MODE><<<1, kTopBS, 0, st>>>(n, h->left, h->right, h->parent, h->range,     \
-                                              h->left_at_split, h->arrive, h->mids,          \
-                                              h->counts, h->agg, wo);                 \
+        k_agg_upper<MODE><<<gu, 256, 0, st>>>(n, h->right, h->range, h->left_at_split,       \
+                                              h->uppers, h->counts, h->trank, h->tsorted,    \
+                                              h->agg, wo);                                   \
     } while (0)
     if (fp64)
         RG_AGG_LAUNCH(0);
@@ -1761,9 +1860,20 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
         const int mode = g_walk_mode >= 0 ? g_walk_mode : 6;
         h->walk_used = mode;
         if (mode == 6) {
-            k_walk32_chunked<4><<<(n + 63) / 64, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
-                                                               out, n, static_cast<float>(eps2),
-                                                               h->stats);
+            // Measured on disk + bulge galaxies, 60k .. 1M particles: octave
+            // chunks (growth 2) until ~700k, then growth 4 (fewer ancestor
+            // tests once the walk is throughput-bound); 8 waves per group only
+            // pay below ~100k, where the groups are too few to fill the GPU.
+            const int shift = g_chunk_shift ? g_chunk_shift : (n < 700000 ? 1 : 2);
+            const int waves = g_chunk_waves ? g_chunk_waves : (n < 100000 ? 8 : 4);
+            const ChunkShape cs{g_chunk_dmin, shift, g_chunk_run};
+            const int ng = (n + 63) / 64;
+            if (waves == 8)
+                k_walk32_chunked<8><<<ng, 512, 0, st>>>(h->recA, h->recB, h->pre, order, out, n,
+                                                        static_cast<float>(eps2), h->stats, cs);
+            else
+                k_walk32_chunked<4><<<ng, 256, 0, st>>>(h->recA, h->recB, h->pre, order, out, n,
+                                                        static_cast<float>(eps2), h->stats, cs);
         } else if (mode == 5) {
             k_walk32_wave_split<4><<<(n + 63) / 64, 256, 0, st>>>(h->recA, h->recB, h->pre,
                                                                   order, out, n,

@@ -70,6 +70,7 @@ extern "C" int rg_set_param(const char *name, int64_t value)
     if (std::strcmp(name, "fp32c_d64") == 0) return rg::rg_direct_set_fp32c_d64_impl(value);
     if (std::strcmp(name, "walk_block") == 0) return rg::rg_bh_set_walk_block_impl(value);
     if (std::strcmp(name, "walk_mode") == 0) return rg::rg_bh_set_walk_mode_impl(value);
+    if (std::strncmp(name, "chunk_", 6) == 0) return rg::rg_bh_set_chunk_impl(name + 6, value);
     rg::set_error("unknown parameter '%s'", name);
     return RG_ERR_ARG;
 }

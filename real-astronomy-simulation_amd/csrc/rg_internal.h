@@ -24,6 +24,7 @@ int rg_direct_set_ipt_impl(int64_t v);
 int rg_direct_set_fp32c_d64_impl(int64_t v);
 int rg_bh_set_walk_block_impl(int64_t v);
 int rg_bh_set_walk_mode_impl(int64_t v);
+int rg_bh_set_chunk_impl(const char *what, int64_t v);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
