@@ -1125,12 +1125,10 @@ __global__ __launch_bounds__(256) void k_walk32_wave(const float4 *__restrict__ 
                 next = k + 1;
             }
         }
-        // every lane's next slot is > k; the smallest is k + 1 whenever a
-        // lane opened node k
-        if (__ballot(next == k + 1))
-            k = k + 1;
-        else
-            k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+        // Every lane not at k waits at the skip slot of an ancestor of k,
+        // which is >= skip(k); so the smallest next slot is k + 1 if a lane
+        // opened node k, else skip(k), with no reduction over the lanes.
+        k = __ballot(next == k + 1) ? k + 1 : max(skip, k + 1);
     }
     if (active) {
         const int p = order[t];
@@ -1350,10 +1348,9 @@ __global__ __launch_bounds__(256) void k_walk32_wave_split(
                 }
             }
         }
-        if (__ballot(next == k + 1))
-            k = k + 1;
-        else
-            k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+        // the smallest next slot: k + 1 if a lane opened k, else skip(k)
+        // (every other lane waits at an ancestor's skip, >= skip(k))
+        k = __ballot(next == k + 1) ? k + 1 : skip;
     }
     if (part > 0) {
         s_part[grp][part - 1][lane][0] = ax;
@@ -1533,10 +1530,9 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     }
                 }
             }
-            if (__ballot(next == k + 1))
-                k = k + 1;
-            else
-                k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+            // the smallest next slot: k + 1 if a lane opened k, else skip(k)
+            // (every other lane waits at an ancestor's skip, >= skip(k))
+            k = __ballot(next == k + 1) ? k + 1 : skip;
         }
         s_part[c][0][lane] = ax;
         s_part[c][1][lane] = ay;
