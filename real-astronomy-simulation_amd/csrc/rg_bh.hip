@@ -7,9 +7,10 @@
 //
 //   1. k_bbox_*    FP64 bounding box (exact min/max, fixed-order reduction)
 //   2. k_morton    keys bit-identical to morton_codes (barnes_hut.py:38-46)
-//   3. k_onesweep  stable LSD radix sort of (key, index), 8-bit digits, one
-//                  kernel per digit with decoupled look-back; the
-//                  permutation equals np.argsort(kind="stable") (:247)
+//   3. k_onesweep  stable LSD radix sort of (key, index), 8-bit digits (8
+//                  passes for 63-bit keys), one kernel per digit with
+//                  decoupled look-back; the permutation equals
+//                  np.argsort(kind="stable") (:247)
 //   4. k_build     Karras build with the reference's exact delta (incl. the
 //                  index tie-break) using hardware clz (:50-121)
 //   5. k_agg_*     leaf init + deterministic aggregation (:124-177), every
@@ -17,7 +18,7 @@
 //                  treelets of <= 256 leaves bottom-up in LDS (the second
 //                  child to finish combines both), then every node above
 //                  them on its own as the sum of its run of treelets, in a
-//                  fixed order (k_treelet_rank orders the treelets).
+//                  fixed order (k_rank_preorder orders the treelets).
 //   6. k_walk*     stackless walk in right-first DFS order: the node
 //                  sequence the reference's stack walk pops (:180-217).
 //                  FP64 follows right-child/escape links over node-id
@@ -44,7 +45,12 @@ namespace {
 constexpr int kSortBS = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortBS * kSortItems;  // keys per sort tile
-constexpr int kMaxPasses = 8;                    // 8-bit digits of a 64-bit key
+// Radix digit width.  9-bit digits (7 passes for a 63-bit key) measured
+// slower than 8-bit ones (8 passes): 15.3 against 12.1 us per pass at 120k.
+constexpr int kDigitBits = 8;
+constexpr int kBuckets = 1 << kDigitBits;         // 256
+constexpr unsigned kDigitMask = kBuckets - 1u;
+constexpr int kMaxPasses = (63 + kDigitBits - 1) / kDigitBits;  // 8 for a 63-bit key
 constexpr int kBoxBS = 256;
 constexpr int kWalkBSDefault = 256;
 
@@ -124,11 +130,15 @@ __global__ __launch_bounds__(kBoxBS) void k_bbox_final(const double *__restrict_
                                                         int bits, double *__restrict__ box,
                                                         unsigned long long *__restrict__ stats,
                                                         unsigned *__restrict__ ghist,
-                                                        int *__restrict__ tickets)
+                                                        int *__restrict__ tickets,
+                                                        int *__restrict__ counts)
 {
     if (ghist)  // the sort's global digit histograms and tile tickets start at zero
-        for (int k = threadIdx.x; k < kMaxPasses * 256; k += kBoxBS) ghist[k] = 0u;
+        for (int k = threadIdx.x; k < kMaxPasses * kBuckets; k += kBoxBS) ghist[k] = 0u;
     if (tickets && threadIdx.x < kMaxPasses) tickets[threadIdx.x] = 0;
+    // the tree's list counters (treelets, upper nodes, rank tickets, treelets
+    // in leaf order) start at zero
+    if (counts && threadIdx.x < 4) counts[threadIdx.x] = 0;
     __shared__ double s[7][kBoxBS];
     const int t = threadIdx.x;
     double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
@@ -190,16 +200,17 @@ __device__ __forceinline__ uint64_t part1by2(uint64_t x)
     return x;
 }
 
-// Keys, plus (when passes > 0) the global histogram of every 8-bit digit
+// Keys, plus (when passes > 0) the global histogram of every radix digit
 // for the radix sort and the reset of the sort's look-back flags.
 __global__ __launch_bounds__(256) void k_morton(const double *__restrict__ pos, int n, int bits,
                                                 const double *__restrict__ box,
                                                 uint64_t *__restrict__ keys, int passes,
                                                 unsigned *__restrict__ ghist,
-                                                unsigned *__restrict__ lookback, int lb_words)
+                                                unsigned *__restrict__ lookback, int lb_words,
+                                                int *__restrict__ tl_of)
 {
-    __shared__ unsigned s_h[kMaxPasses * 256];
-    for (int k = threadIdx.x; k < passes * 256; k += blockDim.x) s_h[k] = 0u;
+    __shared__ unsigned s_h[kMaxPasses * kBuckets];
+    for (int k = threadIdx.x; k < passes * kBuckets; k += blockDim.x) s_h[k] = 0u;
     __syncthreads();
     const double lo0 = box[0], lo1 = box[1], lo2 = box[2], scale = box[3];
     const double qmax = static_cast<double>((1ull << bits) - 1ull);
@@ -215,10 +226,12 @@ __global__ __launch_bounds__(256) void k_morton(const double *__restrict__ pos, 
             key |= part1by2(static_cast<uint64_t>(t)) << a;
         }
         keys[i] = key;
-        for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p * 256 + ((key >> (8 * p)) & 0xFFu)], 1u);
+        if (tl_of) tl_of[i] = -1;  // set again by the build for the first leaf of each treelet
+        for (int p = 0; p < passes; ++p)
+            atomicAdd(&s_h[p * kBuckets + ((key >> (kDigitBits * p)) & kDigitMask)], 1u);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < passes * 256; k += blockDim.x)
+    for (int k = threadIdx.x; k < passes * kBuckets; k += blockDim.x)
         if (s_h[k]) atomicAdd(&ghist[k], s_h[k]);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < lb_words; k += gridDim.x * blockDim.x)
         lookback[k] = 0u;
@@ -226,16 +239,18 @@ __global__ __launch_bounds__(256) void k_morton(const double *__restrict__ pos, 
 
 int launch_bbox_morton(const double *pos, int n, int bits, double *part, double *box,
                        unsigned long long *stats, uint64_t *keys, int passes, unsigned *ghist,
-                       int *tickets, unsigned *lookback, int lb_words, hipStream_t st)
+                       int *tickets, unsigned *lookback, int lb_words, int *counts, int *tl_of,
+                       hipStream_t st)
 {
     const int nb = min((n + kBoxBS - 1) / kBoxBS, 1024);
     k_bbox_partial<<<nb, kBoxBS, 0, st>>>(pos, n, part);
     RG_CHECK_LAUNCH();
-    k_bbox_final<<<1, kBoxBS, 0, st>>>(part, nb, bits, box, stats, ghist, tickets);
+    k_bbox_final<<<1, kBoxBS, 0, st>>>(part, nb, bits, box, stats, ghist, tickets, counts);
     RG_CHECK_LAUNCH();
-    // each block flushes passes x 256 global atomics: keep the grid modest
+    // each block flushes passes x kBuckets global atomics: keep the grid modest
     const int mg = min((n + 255) / 256, passes > 0 ? 256 : 2048);
-    k_morton<<<mg, 256, 0, st>>>(pos, n, bits, box, keys, passes, ghist, lookback, lb_words);
+    k_morton<<<mg, 256, 0, st>>>(pos, n, bits, box, keys, passes, ghist, lookback, lb_words,
+                                 tl_of);
     RG_CHECK_LAUNCH();
     return RG_OK;
 }
@@ -247,19 +262,45 @@ size_t bbox_part_doubles(int n)
 }
 
 // --------------------------------------------------------- radix sort
-// Stable LSD radix sort, one kernel per 8-bit digit ("one sweep"): each
-// tile of kSortTile keys takes a ticket (tiles start in ticket order), counts
-// its digits, publishes the counts and looks back over earlier tiles'
-// published counts (decoupled look-back) for its exclusive offsets.  A
-// look-back word packs a status (1 = this tile's count, 2 = inclusive
-// prefix) with a 30-bit count, so a single relaxed agent-scope atomic
-// carries both.  Every wait is on a tile that started earlier and publishes
-// without waiting, and is bounded; exceeding the bound flags stats[3]
-// instead of hanging.
+// Stable LSD radix sort, one kernel per kDigitBits-bit digit ("one sweep"):
+// each tile of kSortTile keys takes a ticket
+// (tiles start in ticket order), counts its digits, publishes the counts and
+// looks back over earlier tiles' published counts (decoupled look-back) for
+// its exclusive offsets.  A look-back word packs a status (1 = this tile's
+// count, 2 = inclusive prefix) with a 30-bit count, so a single relaxed
+// agent-scope atomic carries both.  Every wait is on a tile that started
+// earlier and publishes without waiting, and is bounded; exceeding the bound
+// flags stats[3] instead of hanging.
 constexpr unsigned kFlagAgg = 1u << 30;
 constexpr unsigned kFlagInc = 2u << 30;
 constexpr unsigned kCountMask = (1u << 30) - 1u;
 constexpr int kSpinLimit = 1 << 20;
+
+// Exclusive prefix over earlier tiles of one look-back word column.
+__device__ __forceinline__ unsigned look_back(const unsigned *__restrict__ lb, int tile,
+                                              int stride, unsigned long long *__restrict__ stats)
+{
+    unsigned excl = 0;
+    int j = tile - 1;
+    int spins = 0;
+    while (j >= 0) {
+        const unsigned v = __hip_atomic_load(&lb[j * stride], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned f = v & ~kCountMask;
+        if (f == 0u) {
+            if (++spins > kSpinLimit) {
+                atomicAdd(&stats[3], 1ull);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += v & kCountMask;
+        if (f == kFlagInc) break;
+        --j;
+    }
+    return excl;
+}
 
 __global__ __launch_bounds__(kSortBS) void k_onesweep(
     const uint64_t *__restrict__ kin, const int *__restrict__ iin, uint64_t *__restrict__ kout,
@@ -268,25 +309,41 @@ __global__ __launch_bounds__(kSortBS) void k_onesweep(
     unsigned long long *__restrict__ stats)
 {
     constexpr int kWaves = kSortBS / 64;
-    constexpr int kRounds = kSortItems;  // each wave: kRounds rounds of 64 keys
-    __shared__ unsigned s_base[kWaves][256];
-    __shared__ unsigned s_scan[256];
+    constexpr int kRounds = kSortItems;        // each wave: kRounds rounds of 64 keys
+    constexpr int DPT = kBuckets / kSortBS;    // digits owned per thread (adjacent)
+    __shared__ unsigned s_base[kWaves][kBuckets];
+    __shared__ unsigned s_scan[kSortBS];
     __shared__ int s_tile;
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int lane = t & 63;
     if (t == 0) s_tile = atomicAdd(ticket_p, 1);
-    for (int k = t; k < kWaves * 256; k += kSortBS) (&s_base[0][0])[k] = 0u;
-    s_scan[t] = ghist_p[t];
+    for (int k = t; k < kWaves * kBuckets; k += kSortBS) (&s_base[0][0])[k] = 0u;
+    unsigned gh[DPT];
+    unsigned gsum = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        gh[q] = ghist_p[t * DPT + q];
+        gsum += gh[q];
+    }
+    s_scan[t] = gsum;
     __syncthreads();
     // exclusive scan of this pass's global digit counts
-    for (int off = 1; off < 256; off <<= 1) {
+    for (int off = 1; off < kSortBS; off <<= 1) {
         const unsigned v = t >= off ? s_scan[t - off] : 0u;
         __syncthreads();
         s_scan[t] += v;
         __syncthreads();
     }
-    const unsigned goff = s_scan[t] - ghist_p[t];
+    unsigned goff[DPT];
+    {
+        unsigned run = s_scan[t] - gsum;
+#pragma unroll
+        for (int q = 0; q < DPT; ++q) {
+            goff[q] = run;
+            run += gh[q];
+        }
+    }
     const int tile = s_tile;
 
     const int seg = tile * kSortTile + w * (kRounds * 64);
@@ -298,49 +355,44 @@ __global__ __launch_bounds__(kSortBS) void k_onesweep(
         if (i < n) {
             key[r] = kin[i];
             idv[r] = iin ? iin[i] : i;
-            atomicAdd(&s_base[w][(key[r] >> shift) & 0xFFu], 1u);
+            atomicAdd(&s_base[w][(key[r] >> shift) & kDigitMask], 1u);
         } else {
             key[r] = 0;
             idv[r] = -1;
         }
     }
     __syncthreads();
-    // thread t owns digit t
-    unsigned cnt = 0;
+    // thread t owns digits t * DPT .. t * DPT + DPT - 1: publish all, then look back
+    unsigned cnt[DPT];
 #pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) cnt += s_base[ww][t];
-    unsigned *mine = &lookback_p[tile * 256 + t];
-    __hip_atomic_store(mine, (tile == 0 ? kFlagInc : kFlagAgg) | cnt, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    unsigned excl = 0;
-    if (tile > 0) {
-        int j = tile - 1;
-        int spins = 0;
-        while (j >= 0) {
-            const unsigned v = __hip_atomic_load(&lookback_p[j * 256 + t], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned f = v & ~kCountMask;
-            if (f == 0u) {
-                if (++spins > kSpinLimit) {
-                    atomicAdd(&stats[3], 1ull);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            excl += v & kCountMask;
-            if (f == kFlagInc) break;
-            --j;
-        }
-        __hip_atomic_store(mine, kFlagInc | (excl + cnt), __ATOMIC_RELAXED,
+    for (int q = 0; q < DPT; ++q) {
+        const int d = t * DPT + q;
+        cnt[q] = 0;
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww) cnt[q] += s_base[ww][d];
+        __hip_atomic_store(&lookback_p[tile * kBuckets + d],
+                           (tile == 0 ? kFlagInc : kFlagAgg) | cnt[q], __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    {
-        unsigned run = goff + excl;
+    unsigned excl[DPT];
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        const int d = t * DPT + q;
+        excl[q] = 0;
+        if (tile > 0) {
+            excl[q] = look_back(lookback_p + d, tile, kBuckets, stats);
+            __hip_atomic_store(&lookback_p[tile * kBuckets + d], kFlagInc | (excl[q] + cnt[q]),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        const int d = t * DPT + q;
+        unsigned run = goff[q] + excl[q];
 #pragma unroll
         for (int ww = 0; ww < kWaves; ++ww) {
-            const unsigned c = s_base[ww][t];
-            s_base[ww][t] = run;
+            const unsigned c = s_base[ww][d];
+            s_base[ww][d] = run;
             run += c;
         }
     }
@@ -350,10 +402,10 @@ __global__ __launch_bounds__(kSortBS) void k_onesweep(
     for (int r = 0; r < kRounds; ++r) {
         const int i = seg + r * 64 + lane;
         const bool valid = i < n;
-        const unsigned d = static_cast<unsigned>((key[r] >> shift) & 0xFFu);
+        const unsigned d = static_cast<unsigned>((key[r] >> shift) & kDigitMask);
         unsigned long long peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < kDigitBits; ++b) {
             const unsigned long long bb = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bb : ~bb;
         }
@@ -383,15 +435,27 @@ __device__ __forceinline__ int delta_dev(int64_t i, int64_t j, int64_t n, uint64
 
 // Upward links for the preorder pass: up[v] = (parent id << 1) | (v is the
 // right child); up[root] = -1.
+constexpr int kTreelet = 256;  // max leaves of a treelet (one workgroup, LDS)
+
+// The Karras build, plus the aggregation plan.  Treelets are the maximal
+// subtrees with at most kTreelet leaves (a treelet root has at most kTreelet
+// leaves and its parent more); they partition the leaves.  The thread of
+// internal node i knows the leaf counts of i and of both children, so it
+// lists i as an upper node (more than kTreelet leaves) and each child that
+// is a treelet root; tl_of[first leaf of a treelet] = its root (tl_of was
+// cleared to -1 by k_morton).  Also clears the treelet-rank look-back words.
 __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restrict__ left,
                         int *__restrict__ right, int *__restrict__ parent,
                         int *__restrict__ up, int2 *__restrict__ range,
                         int *__restrict__ left_at_split, int *__restrict__ counts,
-                        int *__restrict__ tl_of)
+                        int *__restrict__ tl_of, int *__restrict__ treelets,
+                        int *__restrict__ uppers, unsigned *__restrict__ rank_lb,
+                        int rank_tiles)
 {
     const int64_t nn = n;
-    for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < n; li += gridDim.x * blockDim.x)
-        tl_of[li] = -1;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rank_tiles;
+         k += gridDim.x * blockDim.x)
+        rank_lb[k] = 0u;
     for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < n - 1;
          ii += gridDim.x * blockDim.x) {
         const int64_t i = ii;
@@ -429,9 +493,20 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
         if (i == 0) {
             parent[nn] = -1;
             up[nn] = -1;
-            counts[0] = 0;  // treelets
-            counts[1] = 0;  // nodes above the treelets
-            counts[2] = 0;  // treelet-rank tile tickets
+        }
+        if (hi - lo + 1 > kTreelet) {
+            uppers[atomicAdd(&counts[1], 1)] = self;
+            if (gamma - lo + 1 <= kTreelet) {
+                treelets[atomicAdd(&counts[0], 1)] = lc;
+                tl_of[lo] = lc;
+            }
+            if (hi - gamma <= kTreelet) {
+                treelets[atomicAdd(&counts[0], 1)] = rc;
+                tl_of[gamma + 1] = rc;
+            }
+        } else if (i == 0) {  // the root is itself a treelet
+            treelets[atomicAdd(&counts[0], 1)] = self;
+            tl_of[0] = self;
         }
     }
 }
@@ -443,11 +518,12 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
 // longer than any valid tree (corrupt links).
 constexpr int kMaxDepth = 128;  // > 63 + 31 + 1, the deepest tree the tie-break allows
 
-__global__ void k_preorder(const int *__restrict__ up, const int2 *__restrict__ range, int n,
-                           int *__restrict__ pre, unsigned long long *__restrict__ stats)
+__device__ void preorder_part(const int *__restrict__ up, const int2 *__restrict__ range, int n,
+                              int *__restrict__ pre, unsigned long long *__restrict__ stats,
+                              int block, int nblocks)
 {
     const int m = 2 * n - 1;
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m; v += gridDim.x * blockDim.x) {
+    for (int v = block * blockDim.x + threadIdx.x; v < m; v += nblocks * blockDim.x) {
         const int hi = v < n ? v : range[v - n].y;
         int r = 0, u = v, steps = 0;
         while (u != n && u >= 0 && u < m && steps < kMaxDepth) {
@@ -523,49 +599,18 @@ __device__ __forceinline__ void write_node(Agg *agg, const WalkOut &wo,
     }
 }
 
-constexpr int kTreelet = 256;  // max leaves of a treelet (one workgroup, LDS)
 constexpr int kRankItems = 16;
 constexpr int kRankTile = 256 * kRankItems;  // leaves per treelet-rank tile
-
-// Treelets: the maximal subtrees with at most kTreelet leaves (a node is a
-// treelet root when its leaf count is <= kTreelet and its parent's is
-// larger); they partition the leaves.  tl_of[first leaf of a treelet] = its
-// root (other leaves keep -1); the internal nodes above the treelets are
-// listed in uppers.  Also clears the treelet-rank look-back words.
-__global__ void k_classify(const int *__restrict__ parent, const int2 *__restrict__ range, int n,
-                           int *__restrict__ treelets, int *__restrict__ uppers,
-                           int *__restrict__ counts, int *__restrict__ tl_of,
-                           unsigned *__restrict__ rank_lb, int rank_tiles)
-{
-    const int m = 2 * n - 1;
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < m; v += gridDim.x * blockDim.x) {
-        if (v < rank_tiles) rank_lb[v] = 0u;
-        const int lo = v < n ? v : range[v - n].x;
-        const int size = v < n ? 1 : range[v - n].y - lo + 1;
-        if (size > kTreelet) {
-            uppers[atomicAdd(&counts[1], 1)] = v;
-            continue;
-        }
-        const int p = parent[v];
-        const int psize = p < 0 ? INT_MAX : range[p - n].y - range[p - n].x + 1;
-        if (psize > kTreelet) {
-            treelets[atomicAdd(&counts[0], 1)] = v;
-            tl_of[lo] = v;
-        }
-    }
-}
 
 // Rank of every treelet in leaf order: an exclusive scan over the leaves of
 // "first leaf of a treelet", one tile of kRankTile leaves per workgroup with
 // decoupled look-back (the sort's scheme: one word per tile, status + count,
 // bounded waits).  trank[first leaf] = rank, tsorted[rank] = root, and
 // counts[3] = the number of treelets.
-__global__ __launch_bounds__(256) void k_treelet_rank(const int *__restrict__ tl_of, int n,
-                                                      int *__restrict__ trank,
-                                                      int *__restrict__ tsorted,
-                                                      unsigned *__restrict__ lb,
-                                                      int *__restrict__ counts,
-                                                      unsigned long long *__restrict__ stats)
+__device__ void treelet_rank_tile(const int *__restrict__ tl_of, int n, int *__restrict__ trank,
+                                  int *__restrict__ tsorted, unsigned *__restrict__ lb,
+                                  int *__restrict__ counts, int ntiles,
+                                  unsigned long long *__restrict__ stats)
 {
     __shared__ int s_tile;
     __shared__ unsigned s_wsum[4];
@@ -626,7 +671,7 @@ __global__ __launch_bounds__(256) void k_treelet_rank(const int *__restrict__ tl
                                __HIP_MEMORY_SCOPE_AGENT);
         }
         s_excl = excl;
-        if (tile == static_cast<int>(gridDim.x) - 1) counts[3] = static_cast<int>(excl + total);
+        if (tile == ntiles - 1) counts[3] = static_cast<int>(excl + total);
     }
     __syncthreads();
     unsigned r0 = s_excl + woff + incl - cnt;
@@ -639,6 +684,22 @@ __global__ __launch_bounds__(256) void k_treelet_rank(const int *__restrict__ tl
             ++r0;
         }
     }
+}
+
+// The first ntiles blocks rank the treelets; the others (when pre is set)
+// compute the preorder slots -- two independent passes in one launch.  A
+// ranking block waits only on ranking blocks that took earlier tickets.
+__global__ __launch_bounds__(256) void k_rank_preorder(
+    const int *__restrict__ tl_of, int n, int *__restrict__ trank, int *__restrict__ tsorted,
+    unsigned *__restrict__ lb, int *__restrict__ counts, int ntiles,
+    unsigned long long *__restrict__ stats, const int *__restrict__ up,
+    const int2 *__restrict__ range, int *__restrict__ pre)
+{
+    if (static_cast<int>(blockIdx.x) < ntiles) {
+        treelet_rank_tile(tl_of, n, trank, tsorted, lb, counts, ntiles, stats);
+        return;
+    }
+    preorder_part(up, range, n, pre, stats, blockIdx.x - ntiles, gridDim.x - ntiles);
 }
 
 // Pass 1: one workgroup per treelet (grid-stride over the list), one
@@ -1578,8 +1639,8 @@ struct rg_bh {
 
     uint64_t *keys[2] = {nullptr, nullptr};
     int *idx[2] = {nullptr, nullptr};
-    unsigned *ghist = nullptr;    // kMaxPasses x 256 global digit counts
-    unsigned *lookback = nullptr; // passes x nblk x 256 look-back words
+    unsigned *ghist = nullptr;    // kMaxPasses x kBuckets global digit counts
+    unsigned *lookback = nullptr; // passes x nblk x kBuckets look-back words
     int *tickets = nullptr;
     int *left = nullptr, *right = nullptr, *parent = nullptr;
     int2 *range = nullptr;
@@ -1694,7 +1755,7 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     const size_t N = static_cast<size_t>(n);
     const size_t M = 2 * N - 1;
     h->nblk = static_cast<int>((N + kSortTile - 1) / kSortTile);
-    h->passes = (3 * morton_bits + 7) / 8;
+    h->passes = (3 * morton_bits + kDigitBits - 1) / kDigitBits;
     int st = RG_OK;
 #define ALLOC(ptr, bytes)                                                  \
     do {                                                                   \
@@ -1711,9 +1772,9 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->keys[1], 8 * N);
     ALLOC(h->idx[0], 4 * N);
     ALLOC(h->idx[1], 4 * N);
-    ALLOC(h->ghist, 4 * 256 * kMaxPasses);
+    ALLOC(h->ghist, 4 * kBuckets * kMaxPasses);
     ALLOC(h->tickets, 4 * kMaxPasses);
-    ALLOC(h->lookback, 4 * 256 * static_cast<size_t>(h->nblk) * h->passes);
+    ALLOC(h->lookback, 4 * kBuckets * static_cast<size_t>(h->nblk) * h->passes);
     ALLOC(h->left, 4 * (N - 1));
     ALLOC(h->right, 4 * (N - 1));
     ALLOC(h->range, 8 * (N - 1));
@@ -1768,18 +1829,20 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     double *box = h->box;
     int rc = bh_mark(h, 0, st);
     if (rc != RG_OK) return rc;
-    const int lb_words = 256 * h->nblk * h->passes;
+    const int lb_words = kBuckets * h->nblk * h->passes;
     rc = launch_bbox_morton(pos, n, h->bits, part, box, h->stats, h->keys[0], h->passes,
-                            h->ghist, h->tickets, h->lookback, lb_words, st);
+                            h->ghist, h->tickets, h->lookback, lb_words, h->counts, h->tl_of,
+                            st);
     if (rc != RG_OK) return rc;
 
-    // Stable LSD radix sort, 8-bit digits, one kernel per pass.
+    // Stable LSD radix sort, kDigitBits-bit digits, one kernel per pass.
     int cur = 0;
     for (int p = 0; p < h->passes; ++p) {
         k_onesweep<<<h->nblk, kSortBS, 0, st>>>(h->keys[cur], p == 0 ? nullptr : h->idx[cur],
-                                                h->keys[cur ^ 1], h->idx[cur ^ 1], n, 8 * p,
-                                                h->ghist + 256 * p,
-                                                h->lookback + static_cast<size_t>(256) * h->nblk * p,
+                                                h->keys[cur ^ 1], h->idx[cur ^ 1], n,
+                                                kDigitBits * p, h->ghist + kBuckets * p,
+                                                h->lookback + static_cast<size_t>(kBuckets) *
+                                                                  h->nblk * p,
                                                 h->tickets + p, h->stats);
         RG_CHECK_LAUNCH();
         cur ^= 1;
@@ -1792,21 +1855,16 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
 
     const int g1 = min((n + 255) / 256, 16384);
     k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->up, h->range,
-                                h->left_at_split, h->counts, h->tl_of);
+                                h->left_at_split, h->counts, h->tl_of, h->treelets, h->uppers,
+                                h->rank_lb, h->rank_tiles);
     RG_CHECK_LAUNCH();
     const bool fp64 = h->precision == RG_FP64;
     {
-        const int gc = min((2 * n - 1 + 255) / 256, 16384);
-        k_classify<<<gc, 256, 0, st>>>(h->parent, h->range, n, h->treelets, h->uppers,
-                                       h->counts, h->tl_of, h->rank_lb, h->rank_tiles);
-        RG_CHECK_LAUNCH();
-        k_treelet_rank<<<h->rank_tiles, 256, 0, st>>>(h->tl_of, n, h->trank, h->tsorted,
-                                                      h->rank_lb, h->counts, h->stats);
-        RG_CHECK_LAUNCH();
-    }
-    if (!fp64) {
-        const int gp = min((2 * n - 1 + 255) / 256, 16384);
-        k_preorder<<<gp, 256, 0, st>>>(h->up, h->range, n, h->pre, h->stats);
+        // FP32 walks also need the preorder slots (blocks after the ranking ones)
+        const int gp = fp64 ? 0 : min((2 * n - 1 + 255) / 256, 16384);
+        k_rank_preorder<<<h->rank_tiles + gp, 256, 0, st>>>(
+            h->tl_of, n, h->trank, h->tsorted, h->rank_lb, h->counts, h->rank_tiles, h->stats,
+            h->up, h->range, h->pre);
         RG_CHECK_LAUNCH();
     }
 
@@ -2027,5 +2085,5 @@ extern "C" int rg_morton_codes(const double *pos, int64_t n, int bits, uint64_t 
     if (st != RG_OK) return st;
     double *box = static_cast<double *>(scratch);
     return launch_bbox_morton(pos, ni, bits, box + 8, box, nullptr, keys, 0, nullptr, nullptr,
-                              nullptr, 0, rg::as_stream(stream));
+                              nullptr, 0, nullptr, nullptr, rg::as_stream(stream));
 }
