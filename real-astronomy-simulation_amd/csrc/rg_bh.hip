@@ -1550,6 +1550,41 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         int k0 = -64;
         float4 wa = make_float4(0.f, 0.f, 0.f, 0.f);
         int2 wb = make_int2(0, 0);
+        // Ancestors of lo (slots before the chunk whose subtree reaches into
+        // it): tested, not counted or added -- an earlier chunk owns them.
+        // Subtrees ending at or before lo are stepped over untested.
+        while (k < lo) {
+            if (k - k0 >= 64) {
+                k0 = k;
+                const int slot = k0 + lane;
+                if (slot < m) {
+                    wa = recA[slot];
+                    wb = recB[slot];
+                }
+            }
+            const int l = k - k0;
+            const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
+            if (skip <= lo) {
+                if (next == k) next = skip;
+            } else {
+                const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
+                const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
+                const float cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
+                const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+                if (next == k) {
+                    const float dx = cx - px;
+                    const float dy = cy - py;
+                    const float dz = cz - pz;
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+                    next = crit < r2 ? skip : k + 1;
+                }
+            }
+            // the smallest next slot: k + 1 if a lane opened k, else skip(k)
+            // (every other lane waits at an ancestor's skip, >= skip(k))
+            k = __ballot(next == k + 1) ? k + 1 : skip;
+        }
+        // The chunk's own slots: every node a lane reaches is tested, counted
+        // and, when accepted, added.
         while (k < hi) {
             if (k - k0 >= 64) {
                 k0 = k;
@@ -1567,32 +1602,22 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
             const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
             if (next == k) {
-                if (skip <= lo) {
-                    next = skip;  // subtree entirely before this chunk
+                ++visits;
+                const float dx = cx - px;
+                const float dy = cy - py;
+                const float dz = cz - pz;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+                if (crit < r2) {
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float s = cw * (inv * inv) * inv;
+                    ax = fmaf(s, dx, ax);
+                    ay = fmaf(s, dy, ay);
+                    az = fmaf(s, dz, az);
+                    next = skip;
                 } else {
-                    const bool own = k >= lo;
-                    const float dx = cx - px;
-                    const float dy = cy - py;
-                    const float dz = cz - pz;
-                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-                    if (crit < r2) {
-                        if (own) {
-                            ++visits;
-                            const float inv = __builtin_amdgcn_rsqf(r2);
-                            const float s = cw * (inv * inv) * inv;
-                            ax = fmaf(s, dx, ax);
-                            ay = fmaf(s, dy, ay);
-                            az = fmaf(s, dz, az);
-                        }
-                        next = skip;
-                    } else {
-                        visits += own ? 1 : 0;
-                        next = k + 1;
-                    }
+                    next = k + 1;
                 }
             }
-            // the smallest next slot: k + 1 if a lane opened k, else skip(k)
-            // (every other lane waits at an ancestor's skip, >= skip(k))
             k = __ballot(next == k + 1) ? k + 1 : skip;
         }
         s_part[c][0][lane] = ax;
