@@ -65,7 +65,7 @@ const char *rg_last_error(void);
  *                 boundary beyond each end of a group (default 256); growth
  *                 factor 2^shift of the next ones (0 = by n: 1 below 700k
  *                 particles, else 2); runs of groups per XCD (32); waves per
- *                 group, 4 or 8 (0 = by n: 8 below 100k particles)
+ *                 group, 4 or 8 (0 = by n: 8 below 150k particles)
  * Process-wide; takes effect at the next launch. */
 int rg_set_param(const char *name, int64_t value);
 

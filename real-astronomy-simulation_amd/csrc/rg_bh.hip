@@ -561,12 +561,13 @@ struct WalkOut {
     int theta_zero;
 };
 
+// pslot: the node's preorder slot (FP32 records; the caller has it at hand).
 template <int MODE>
 __device__ __forceinline__ void write_node(Agg *agg, const WalkOut &wo,
                                            int node, int n, double m, const double mom[3],
                                            const double mn[3], const double mx[3],
                                            const double c[3], int lo, int hi, int right_child,
-                                           int escape)
+                                           int escape, int pslot)
 {
     Agg g;
     g.m = m;
@@ -589,7 +590,7 @@ __device__ __forceinline__ void write_node(Agg *agg, const WalkOut &wo,
         r.escape = escape;
         wo.rec64[node] = r;
     } else {
-        const int p = wo.pre[node];
+        const int p = pslot;
         const float crit = leaf ? -INFINITY
                                 : (wo.theta_zero ? INFINITY
                                                  : static_cast<float>(size * size * wo.inv_theta2));
@@ -705,7 +706,13 @@ __global__ __launch_bounds__(256) void k_rank_preorder(
 // Pass 1: one workgroup per treelet (grid-stride over the list), one
 // thread per leaf.  Every node of the treelet has both children inside it:
 // node data live in LDS and the arrival counters are LDS atomics at
-// workgroup scope (acq_rel: the second arriver sees the first's data).
+// workgroup scope (acq_rel: the second arriver sees the first's data).  A
+// leaf keeps only its position and mass in LDS (its moment m * x is
+// recomputed with the same rounding, its box is the point), so the
+// workgroup needs 29 KB of LDS for the node data.  The topology of the
+// treelet's internal nodes (every internal index lies in the treelet's leaf
+// range) is staged in LDS before the climb, so a level costs LDS accesses
+// and fire-and-forget stores, not dependent global loads.
 template <int MODE>
 __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     const int *__restrict__ order, const double *__restrict__ pos,
@@ -716,11 +723,15 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     WalkOut wo)
 {
     constexpr int C = kTreelet;
-    __shared__ double s_m[2 * C];
-    __shared__ double s_mom[3][2 * C];
-    __shared__ double s_mn[3][2 * C];
-    __shared__ double s_mx[3][2 * C];
+    __shared__ double s_lx[3][C];   // leaves: position
+    __shared__ double s_lm[C];      //         mass
+    __shared__ double s_m[C];       // internal nodes: mass, moment, box
+    __shared__ double s_mom[3][C];
+    __shared__ double s_mn[3][C];
+    __shared__ double s_mx[3][C];
     __shared__ int s_arrive[C];
+    __shared__ int s_L[C], s_R[C], s_par[C], s_pre[C];  // internal node lo + t
+    __shared__ int2 s_rg[C];
     const int tid = threadIdx.x;
     const int cnt = counts[0];
     for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
@@ -728,27 +739,42 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
         const int lo = v < n ? v : range[v - n].x;
         const int size = v < n ? 1 : range[v - n].y - lo + 1;
         s_arrive[tid] = 0;
+        int lpar = -1, lpre = 0;
+        const double *xp = nullptr;
+        double m = 0.0;
+        if (tid < size) {
+            const int ii = lo + tid;  // internal ids run 0 .. n-2
+            if (ii < n - 1) {
+                s_L[tid] = left[ii];
+                s_R[tid] = right[ii];
+                s_rg[tid] = range[ii];
+                s_par[tid] = parent[ii + n];
+                if (MODE != 0) s_pre[tid] = wo.pre[ii + n];
+            }
+            const int p = order[ii];
+            xp = pos + 3 * p;
+            m = mass[p];
+            lpar = parent[ii];
+            if (MODE != 0) lpre = wo.pre[ii];
+        }
         __syncthreads();
         if (tid < size) {
             const int li = lo + tid;
-            const int p = order[li];
-            const double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
-            const double m = mass[p];
+            const double x[3] = {xp[0], xp[1], xp[2]};
             // barnes_hut.py:129-134 (leaf init).
             double mom[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 mom[a] = m * x[a];
-                s_mom[a][tid] = mom[a];
-                s_mn[a][tid] = x[a];
-                s_mx[a][tid] = x[a];
+                s_lx[a][tid] = x[a];
             }
-            s_m[tid] = m;
+            s_lm[tid] = m;
             write_node<MODE>(agg, wo, li, n, m, mom, x, x, x, li, li, -1,
-                             li == 0 ? -1 : left_at_split[li - 1]);
-            int node = li == v ? -1 : parent[li];
+                             li == 0 ? -1 : left_at_split[li - 1], lpre);
+            int node = li == v ? -1 : lpar;
             while (node >= 0) {
                 const int ii = node - n;
+                const int t = ii - lo;
                 // Only the LDS node data must be ordered with the arrival
                 // counter (the global outputs are read by later kernels), so
                 // the fences are LDS-only and do not wait for global stores.
@@ -757,21 +783,56 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
                 if (old == 0) break;  // the sibling subtree is not finished; its thread continues
-                const int L = left[ii], R = right[ii];
-                const int2 rg = range[ii];
-                const int sL = L < n ? L - lo : C + (L - n - lo);
-                const int sR = R < n ? R - lo : C + (R - n - lo);
-                const double mm = s_m[sL] + s_m[sR];
+                const int L = s_L[t], R = s_R[t];
+                const int2 rg = s_rg[t];
+                double mL, mR, momL[3], momR[3], mnL[3], mnR[3], mxL[3], mxR[3];
+                if (L < n) {
+                    const int sL = L - lo;
+                    mL = s_lm[sL];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        momL[a] = mL * s_lx[a][sL];
+                        mnL[a] = mxL[a] = s_lx[a][sL];
+                    }
+                } else {
+                    const int sL = L - n - lo;
+                    mL = s_m[sL];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        momL[a] = s_mom[a][sL];
+                        mnL[a] = s_mn[a][sL];
+                        mxL[a] = s_mx[a][sL];
+                    }
+                }
+                if (R < n) {
+                    const int sR = R - lo;
+                    mR = s_lm[sR];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        momR[a] = mR * s_lx[a][sR];
+                        mnR[a] = mxR[a] = s_lx[a][sR];
+                    }
+                } else {
+                    const int sR = R - n - lo;
+                    mR = s_m[sR];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        momR[a] = s_mom[a][sR];
+                        mnR[a] = s_mn[a][sR];
+                        mxR[a] = s_mx[a][sR];
+                    }
+                }
+                const double mm = mL + mR;
                 double mn[3], mx[3], c[3];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    mom[a] = s_mom[a][sL] + s_mom[a][sR];
-                    mn[a] = fmin(s_mn[a][sL], s_mn[a][sR]);
-                    mx[a] = fmax(s_mx[a][sL], s_mx[a][sR]);
+                    mom[a] = momL[a] + momR[a];
+                    mn[a] = fmin(mnL[a], mnR[a]);
+                    mx[a] = fmax(mxL[a], mxR[a]);
                     // barnes_hut.py:175-177: com = mom / mass where mass > 0.
                     c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
                 }
-                const int sn = C + ii - lo;
+                const int sn = ii - lo;
                 s_m[sn] = mm;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
@@ -780,8 +841,9 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
                     s_mx[a][sn] = mx[a];
                 }
                 write_node<MODE>(agg, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
-                                 rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
-                node = node == v ? -1 : parent[node];
+                                 rg.x == 0 ? -1 : left_at_split[rg.x - 1],
+                                 MODE == 0 ? 0 : s_pre[t]);
+                node = node == v ? -1 : s_par[t];
             }
         }
         __syncthreads();  // the LDS slots are reused by the next treelet
@@ -862,7 +924,8 @@ __global__ __launch_bounds__(256) void k_agg_upper(
                 cm[c] = mm > 0.0 ? mom[c] / mm : 0.0;
             }
             write_node<MODE>(agg, wo, v, n, mm, mom, mn, mx, cm, rg.x, rg.y, right[ii],
-                             rg.x == 0 ? -1 : left_at_split[rg.x - 1]);
+                             rg.x == 0 ? -1 : left_at_split[rg.x - 1],
+                             MODE == 0 ? 0 : wo.pre[v]);
         }
         __syncthreads();  // s_r is reused by the next node
     }
@@ -1942,9 +2005,9 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
             // Measured on disk + bulge galaxies, 60k .. 1M particles: octave
             // chunks (growth 2) until ~700k, then growth 4 (fewer ancestor
             // tests once the walk is throughput-bound); 8 waves per group only
-            // pay below ~100k, where the groups are too few to fill the GPU.
+            // pay below ~150k, where the groups are too few to fill the GPU.
             const int shift = g_chunk_shift ? g_chunk_shift : (n < 700000 ? 1 : 2);
-            const int waves = g_chunk_waves ? g_chunk_waves : (n < 100000 ? 8 : 4);
+            const int waves = g_chunk_waves ? g_chunk_waves : (n < 150000 ? 8 : 4);
             const ChunkShape cs{g_chunk_dmin, shift, g_chunk_run};
             const int ng = (n + 63) / 64;
             if (waves == 8)
