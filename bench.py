@@ -393,6 +393,47 @@ def bench_engines(n_gas=65536, n_tree=120000, reps=5):
     return out
 
 
+def bench_grid(reps=20):
+    """SPH neighbour-grid rebuild (core/solvers/neighbor_grid.py:52-92) on the
+    disk + bulge N=120000 with half the particles gas: the device rebuild
+    (which returns the gas count to the host, as the reference does) against
+    the reference's numpy binning restated in the oracle, on the host."""
+    import numpy as np
+    import torch
+
+    from oracle import np_oracle as orc
+    from rgsim.ic import make_disk_bulge
+    from rgsim.solvers.neighbor_grid import NeighborGrid
+
+    s = make_disk_bulge(n_disk=100000, n_bulge=20000, seed=1)
+    rng = np.random.default_rng(3)
+    h = 0.1 * np.cbrt(rng.uniform(0.2, 1.0, s.n))
+    st = (rng.random(s.n) < 0.5).astype(np.int32)
+    pos = torch.from_numpy(s.pos).cuda()
+    ht = torch.from_numpy(h).cuda()
+    stt = torch.from_numpy(st).cuda()
+    g = NeighborGrid(s.n)
+    g.rebuild(pos, ht, stt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.rebuild(pos, ht, stt)
+    wall = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(3):
+        orc.grid_rebuild(s.pos, h, st)
+    host = (time.perf_counter() - t0) / 3
+    # algorithmic bytes: pos, h, is_star read twice (bounds, keys); keys + index
+    # through the sort passes (read and write, 12 B each); gsort and cell_start
+    passes = 3
+    nbytes = s.n * (2 * (24 + 8 + 4) + passes * 2 * 12 + 4) + 4 * (g.max_cells + 1)
+    out = {"n": s.n, "ngas": g.ngas, "cells": g.nx * g.ny * g.nz, "ms_per_rebuild": 1e3 * wall,
+           "host_numpy_ms": 1e3 * host, "speedup_vs_host": host / wall,
+           "algorithmic_bytes": nbytes, "GB_per_s": nbytes / wall / 1e9}
+    log(f"grid: rebuild {out['ms_per_rebuild']:.3f} ms (host numpy {out['host_numpy_ms']:.1f} ms)")
+    return out
+
+
 def copy_calibration(mib=1024, reps=3):
     """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
     import torch
@@ -406,32 +447,43 @@ def copy_calibration(mib=1024, reps=3):
                       "reps": reps}))
 
 
-def cpu_baseline(steps=2):
+def cpu_baseline(steps=10):
     """The oracle's C/OpenMP restatement of the reference BH step on the host
-    cores: Morton + stable sort + build + aggregation + stack walk + KDK."""
+    cores: Morton + stable sort + build + aggregation + stack walk + KDK.
+    `steps` KDK steps on all threads (about 1 s of wall time, 15-20 s of CPU
+    work on 16 threads), plus one step on a single thread."""
     from oracle import np_oracle as orc
     from rgsim.ic import make_disk_bulge
     from rgsim.units import G
 
     orc.build()
     threads = os.cpu_count() or 1
-    orc.set_threads(threads)
     s = make_disk_bulge(n_disk=100000, n_bulge=20000, seed=1)
-    pos, vel, mass = s.pos.copy(), s.vel.copy(), s.mass
-    acc, _, _ = orc.bh_acc(pos, mass, G, 0.5, 0.05)
-    dt = 1e-4
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        vel = orc.kick(vel, acc, 0.5 * dt)
-        pos = orc.drift(pos, vel, dt)
+
+    def run(nsteps, nthreads):
+        orc.set_threads(nthreads)
+        pos, vel, mass = s.pos.copy(), s.vel.copy(), s.mass
         acc, _, _ = orc.bh_acc(pos, mass, G, 0.5, 0.05)
-        vel = orc.kick(vel, acc, 0.5 * dt)
-    wall = time.perf_counter() - t0
+        dt = 1e-4
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            vel = orc.kick(vel, acc, 0.5 * dt)
+            pos = orc.drift(pos, vel, dt)
+            acc, _, _ = orc.bh_acc(pos, mass, G, 0.5, 0.05)
+            vel = orc.kick(vel, acc, 0.5 * dt)
+        return time.perf_counter() - t0
+
+    wall = run(steps, threads)
+    cores = orc.get_threads()
+    wall1 = run(1, 1)
+    orc.set_threads(threads)
     return {"value": s.n * steps / wall / 1e6, "unit": "Mparticle-steps/s",
-            "cores": orc.get_threads(), "kind": "port",
+            "cores": cores, "kind": "port",
             "sample": f"{steps} KDK steps of the FP64 BH oracle (C/OpenMP restatement of "
                       f"barnes_hut.py + leapfrog.py) on the same N={s.n} disk+bulge, "
-                      f"{wall:.1f} s"}
+                      f"{wall:.1f} s on {cores} threads",
+            "single_thread": {"value": s.n / wall1 / 1e6, "unit": "Mparticle-steps/s",
+                              "sample": f"1 KDK step on 1 thread, {wall1:.1f} s"}}
 
 
 def main():
@@ -443,8 +495,8 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the BH drop-in workload (no direct/adjoint/batch/1M sections)")
-    ap.add_argument("--sections", default="bh,direct,adjoint,batch,bh1m,engines",
-                    help="comma-separated subset of bh,direct,adjoint,batch,bh1m,engines")
+    ap.add_argument("--sections", default="bh,direct,adjoint,batch,bh1m,engines,grid",
+                    help="comma-separated subset of bh,direct,adjoint,batch,bh1m,engines,grid")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -465,7 +517,7 @@ def main():
         sections &= {"bh"}
     if args.no_direct:
         sections.discard("direct")
-    bh = direct = adjoint = batch = bh1m = engines = None
+    bh = direct = adjoint = batch = bh1m = engines = grid = None
     if "bh" in sections:
         log("bench: BH disk+bulge N=120000 ...")
         bh = bench_bh(args.steps, args.warmup)
@@ -485,10 +537,14 @@ def main():
     if "engines" in sections:
         log("bench: engine self-gravity ...")
         engines = bench_engines()
+    if "grid" in sections:
+        log("bench: SPH neighbour grid ...")
+        grid = bench_grid()
     if bh is None:
         print(json.dumps({"sections": sorted(sections), "direct_plummer_262144": direct,
                           "adjoint_plummer_16384": adjoint, "batch_512_systems": batch,
-                          "bh_disk_bulge_1M": bh1m, "engines": engines}))
+                          "bh_disk_bulge_1M": bh1m, "engines": engines,
+                          "neighbor_grid": grid}))
         return
     cpu = None
     if not args.no_cpu_baseline:
@@ -522,6 +578,7 @@ def main():
         "batch_512_systems": batch,
         "bh_disk_bulge_1M": bh1m,
         "engines": engines,
+        "neighbor_grid": grid,
     }
     print(json.dumps(line))
 

@@ -222,6 +222,44 @@ int rg_bh_timing(rg_bh *h, double *ms4, int64_t *count);
 
 int rg_bh_destroy(rg_bh *h);
 
+/* ------------------------------------------------------ SPH neighbour grid */
+
+/* Replaces core.solvers.neighbor_grid.NeighborGrid (core/solvers/
+ * neighbor_grid.py:36-136): a uniform grid over the gas particles whose cell
+ * size is the SPH search radius.  The reference bins on the host in numpy
+ * and uploads; here every step runs on the device, bit-identical. */
+typedef struct rg_grid rg_grid;
+
+typedef struct {
+    int64_t ngas;            /* gas particles binned (0: no grid, outputs untouched) */
+    int64_t nx, ny, nz;      /* grid shape */
+    int64_t n_cells;         /* nx * ny * nz */
+    int64_t max_cells;       /* cell budget, max(64, 4 n) (neighbor_grid.py:42) */
+    double lo[3];            /* grid origin: min gas position per axis */
+    double cell;             /* cell size */
+    double inv_cell;         /* 1 / cell */
+} rg_grid_info_t;
+
+/* NeighborGrid(n, min_radius) (neighbor_grid.py:39-50).  n >= 0. */
+int rg_grid_create(int64_t n, double min_radius, rg_grid **out);
+
+/* NeighborGrid.rebuild(pos, h, is_star) (neighbor_grid.py:52-92), or
+ * rebuild_all(pos, h) (:94-136) when is_star is NULL.  pos (n,3) and h (n)
+ * FP64, is_star (n) int32 (0 = gas), all device pointers.  Writes gsort (n:
+ * gas indices in cell order, stable, 0 past ngas) and cell_start
+ * (max_cells + 1: first gsort position of each cell, 0 past n_cells), and
+ * fills *info (host).  Synchronises the stream, as the reference's rebuild
+ * returns the gas count to the host.  A gas particle with a non-finite
+ * position or smoothing length is rejected (RG_ERR_NONFINITE); the
+ * reference would bin garbage. */
+int rg_grid_rebuild(rg_grid *g, const double *pos, const double *h, const int32_t *is_star,
+                    int32_t *gsort, int32_t *cell_start, rg_grid_info_t *info, void *stream);
+
+/* The cell budget, for sizing cell_start (max_cells + 1 entries). */
+int rg_grid_max_cells(const rg_grid *g, int64_t *max_cells);
+
+int rg_grid_destroy(rg_grid *g);
+
 /* --------------------------------------------------------------- adjoint */
 
 /* Vector-Jacobian product of the direct acceleration (no reference

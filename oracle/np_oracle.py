@@ -32,6 +32,14 @@ Two layers:
   traverse                    theta = 0 reproduces the direct sum exactly in
                               set (every leaf visited), FP64 direct-sum error
                               at theta > 0
+  grid_rebuild                the binning invariants (every gas particle once,
+                              each in the cell that contains it, cells in
+                              order, stable within a cell, cell_start the
+                              cumulative counts), the grid-shape rules
+                              (cell >= 2 h_max and >= min_radius, budget,
+                              growth by 1.3), hand-computed small grids, and
+                              brute-force completeness of the 3x3x3 stencil
+                              for every pair within the search radius
   grav_self_direct,           the direct sum plus the incoming acc, bitwise
   grav_ext_add                (same j loop); the analytic part as below
   analytic_add                exact gradients of the Miyamoto-Nagai, Plummer,
@@ -420,3 +428,49 @@ def two_body_acc(m1, m2, sep, G, eps2):
 
 def two_body_pe(m1, m2, sep, G, eps2):
     return -G * m1 * m2 / np.sqrt(sep * sep + eps2)
+
+
+# ------------------------------------------------------- SPH neighbour grid
+def grid_rebuild(pos, h, is_star=None, min_radius=0.0, n=None):
+    """NeighborGrid.rebuild / rebuild_all (core/solvers/neighbor_grid.py:52-136)
+    restated in numpy.  Returns None when there is no gas (the reference
+    returns 0 and leaves its fields untouched), else a dict with gsort (n,
+    zero-padded past ngas), cell_start (max_cells + 1, zero-padded past
+    n_cells), ngas, dims, lo, cell, inv_cell, max_cells."""
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    h = np.ascontiguousarray(h, np.float64).reshape(-1)
+    n = pos.shape[0] if n is None else n
+    max_cells = max(64, 4 * n)                                    # :42
+    if is_star is None:                                           # :96-98 (rebuild_all)
+        gas = np.arange(n, dtype=np.int32)
+    else:                                                         # :57 (rebuild)
+        gas = np.nonzero(np.asarray(is_star) == 0)[0].astype(np.int32)
+    ngas = int(gas.size)
+    if ngas == 0:
+        return None
+    gpos = pos[gas]
+    h_max = float(h[gas].max())                                   # :63
+    cell = max(2.0 * h_max, float(min_radius))                    # :64
+    cell = cell if cell > 1e-9 else 1.0                           # :65
+    lo = gpos.min(axis=0)                                         # :67
+    span = gpos.max(axis=0) - lo
+    dims = np.maximum(np.floor(span / cell).astype(np.int64) + 1, 1)
+    while int(dims.prod()) > max_cells:                           # :71-73
+        cell *= 1.3
+        dims = np.maximum(np.floor(span / cell).astype(np.int64) + 1, 1)
+    n_cells = int(dims.prod())
+    ijk = np.clip(((gpos - lo) / cell).astype(np.int64), 0, dims - 1)   # :76
+    flat = (ijk[:, 0] * dims[1] + ijk[:, 1]) * dims[2] + ijk[:, 2]
+    order = np.argsort(flat, kind="stable")
+    sorted_global = gas[order]
+    counts = np.bincount(flat, minlength=n_cells)
+    starts = np.empty(n_cells + 1, np.int32)
+    starts[0] = 0
+    starts[1:] = np.cumsum(counts).astype(np.int32)
+    gsort = np.zeros(n, np.int32)                                 # :82-84 (padded)
+    gsort[:ngas] = sorted_global
+    cs = np.zeros(max_cells + 1, np.int32)                        # :85-86
+    cs[:n_cells + 1] = starts
+    return dict(gsort=gsort, cell_start=cs, ngas=ngas, dims=tuple(int(d) for d in dims),
+                lo=tuple(float(x) for x in lo), cell=cell, inv_cell=1.0 / cell,
+                max_cells=max_cells, n_cells=n_cells, flat=flat, gas=gas)

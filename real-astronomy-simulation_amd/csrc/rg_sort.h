@@ -1,0 +1,181 @@
+// Stable LSD radix sort of (key, index) pairs, shared by the Barnes-Hut
+// tree (Morton keys) and the SPH neighbour grid (cell keys).  Device code
+// only; each translation unit that includes it gets its own copy of the
+// kernels.
+#pragma once
+
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace {
+
+constexpr int kSortBS = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortBS * kSortItems;  // keys per sort tile
+// Radix digit width.  9-bit digits (7 passes for a 63-bit key) measured
+// slower than 8-bit ones (8 passes): 15.3 against 12.1 us per pass at 120k.
+constexpr int kDigitBits = 8;
+constexpr int kBuckets = 1 << kDigitBits;         // 256
+constexpr unsigned kDigitMask = kBuckets - 1u;
+constexpr int kMaxPasses = (63 + kDigitBits - 1) / kDigitBits;  // 8 for a 63-bit key
+
+// Stable LSD radix sort, one kernel per kDigitBits-bit digit ("one sweep"):
+// each tile of kSortTile keys takes a ticket
+// (tiles start in ticket order), counts its digits, publishes the counts and
+// looks back over earlier tiles' published counts (decoupled look-back) for
+// its exclusive offsets.  A look-back word packs a status (1 = this tile's
+// count, 2 = inclusive prefix) with a 30-bit count, so a single relaxed
+// agent-scope atomic carries both.  Every wait is on a tile that started
+// earlier and publishes without waiting, and is bounded; exceeding the bound
+// flags stats[3] instead of hanging.
+constexpr unsigned kFlagAgg = 1u << 30;
+constexpr unsigned kFlagInc = 2u << 30;
+constexpr unsigned kCountMask = (1u << 30) - 1u;
+constexpr int kSpinLimit = 1 << 20;
+
+// Exclusive prefix over earlier tiles of one look-back word column.
+__device__ __forceinline__ unsigned look_back(const unsigned *__restrict__ lb, int tile,
+                                              int stride, unsigned long long *__restrict__ stats)
+{
+    unsigned excl = 0;
+    int j = tile - 1;
+    int spins = 0;
+    while (j >= 0) {
+        const unsigned v = __hip_atomic_load(&lb[j * stride], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned f = v & ~kCountMask;
+        if (f == 0u) {
+            if (++spins > kSpinLimit) {
+                atomicAdd(&stats[3], 1ull);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += v & kCountMask;
+        if (f == kFlagInc) break;
+        --j;
+    }
+    return excl;
+}
+
+__global__ __launch_bounds__(kSortBS) void k_onesweep(
+    const uint64_t *__restrict__ kin, const int *__restrict__ iin, uint64_t *__restrict__ kout,
+    int *__restrict__ iout, int n, int shift, const unsigned *__restrict__ ghist_p,
+    unsigned *__restrict__ lookback_p, int *__restrict__ ticket_p,
+    unsigned long long *__restrict__ stats)
+{
+    constexpr int kWaves = kSortBS / 64;
+    constexpr int kRounds = kSortItems;        // each wave: kRounds rounds of 64 keys
+    constexpr int DPT = kBuckets / kSortBS;    // digits owned per thread (adjacent)
+    __shared__ unsigned s_base[kWaves][kBuckets];
+    __shared__ unsigned s_scan[kSortBS];
+    __shared__ int s_tile;
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    const int lane = t & 63;
+    if (t == 0) s_tile = atomicAdd(ticket_p, 1);
+    for (int k = t; k < kWaves * kBuckets; k += kSortBS) (&s_base[0][0])[k] = 0u;
+    unsigned gh[DPT];
+    unsigned gsum = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        gh[q] = ghist_p[t * DPT + q];
+        gsum += gh[q];
+    }
+    s_scan[t] = gsum;
+    __syncthreads();
+    // exclusive scan of this pass's global digit counts
+    for (int off = 1; off < kSortBS; off <<= 1) {
+        const unsigned v = t >= off ? s_scan[t - off] : 0u;
+        __syncthreads();
+        s_scan[t] += v;
+        __syncthreads();
+    }
+    unsigned goff[DPT];
+    {
+        unsigned run = s_scan[t] - gsum;
+#pragma unroll
+        for (int q = 0; q < DPT; ++q) {
+            goff[q] = run;
+            run += gh[q];
+        }
+    }
+    const int tile = s_tile;
+
+    const int seg = tile * kSortTile + w * (kRounds * 64);
+    uint64_t key[kRounds];
+    int idv[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = seg + r * 64 + lane;
+        if (i < n) {
+            key[r] = kin[i];
+            idv[r] = iin ? iin[i] : i;
+            atomicAdd(&s_base[w][(key[r] >> shift) & kDigitMask], 1u);
+        } else {
+            key[r] = 0;
+            idv[r] = -1;
+        }
+    }
+    __syncthreads();
+    // thread t owns digits t * DPT .. t * DPT + DPT - 1: publish all, then look back
+    unsigned cnt[DPT];
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        const int d = t * DPT + q;
+        cnt[q] = 0;
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww) cnt[q] += s_base[ww][d];
+        __hip_atomic_store(&lookback_p[tile * kBuckets + d],
+                           (tile == 0 ? kFlagInc : kFlagAgg) | cnt[q], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    unsigned excl[DPT];
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        const int d = t * DPT + q;
+        excl[q] = 0;
+        if (tile > 0) {
+            excl[q] = look_back(lookback_p + d, tile, kBuckets, stats);
+            __hip_atomic_store(&lookback_p[tile * kBuckets + d], kFlagInc | (excl[q] + cnt[q]),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        const int d = t * DPT + q;
+        unsigned run = goff[q] + excl[q];
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww) {
+            const unsigned c = s_base[ww][d];
+            s_base[ww][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int i = seg + r * 64 + lane;
+        const bool valid = i < n;
+        const unsigned d = static_cast<unsigned>((key[r] >> shift) & kDigitMask);
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kDigitBits; ++b) {
+            const unsigned long long bb = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        const unsigned rank = __popcll(peers & lt_mask);
+        const unsigned base = s_base[w][d];
+        if (valid) {
+            const unsigned dst = min(base + rank, static_cast<unsigned>(n - 1));
+            kout[dst] = key[r];
+            iout[dst] = idv[r];
+            if (rank == 0) s_base[w][d] = base + static_cast<unsigned>(__popcll(peers));
+        }
+    }
+}
+
+}  // namespace

@@ -41,6 +41,14 @@ class Potential(ctypes.Structure):
                 ("smbh_mass", ctypes.c_double), ("is_hernquist", ctypes.c_int32)]
 
 
+class GridInfo(ctypes.Structure):
+    """rg_grid_info_t: the geometry of a neighbour-grid rebuild."""
+    _fields_ = [("ngas", ctypes.c_int64), ("nx", ctypes.c_int64), ("ny", ctypes.c_int64),
+                ("nz", ctypes.c_int64), ("n_cells", ctypes.c_int64),
+                ("max_cells", ctypes.c_int64), ("lo", ctypes.c_double * 3),
+                ("cell", ctypes.c_double), ("inv_cell", ctypes.c_double)]
+
+
 # name -> (restype, argtypes); every symbol include/rgsim.h declares.
 SIGNATURES = {
     "rg_version": (_i, []),
@@ -68,6 +76,10 @@ SIGNATURES = {
     "rg_bh_set_timing": (_i, [_vp, _i]),
     "rg_bh_timing": (_i, [_vp, ctypes.POINTER(_d), ctypes.POINTER(_i64)]),
     "rg_bh_destroy": (_i, [_vp]),
+    "rg_grid_create": (_i, [_i64, _d, ctypes.POINTER(_vp)]),
+    "rg_grid_rebuild": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(GridInfo), _vp]),
+    "rg_grid_max_cells": (_i, [_vp, ctypes.POINTER(_i64)]),
+    "rg_grid_destroy": (_i, [_vp]),
     "rg_direct_vjp": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
     "rg_kdk_adjoint_update": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _vp]),
     "rg_batch_create": (_i, [_i64, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(_vp)]),
