@@ -37,7 +37,7 @@ WALK_RECORD_BYTES = 32          # FP32 node record fetched per visit
 # FP32 walk variant (rg_bh_stats walk_mode) -> the kernel it launches
 WALK_KERNELS = {0: "k_walk32", 1: "k_walk32_pair", 2: "k_walk32_wave", 3: "k_walk32_wave2",
                 4: "k_walk32_wave_split<2>", 5: "k_walk32_wave_split<4>",
-                6: "k_walk32_chunked<4>", -1: "k_walk64"}
+                6: "k_walk32_chunked<{w}>", -1: "k_walk64"}
 WALK_PARTICLE_BYTES = 16 + 4 + 24  # own leaf record + order index + FP64 acceleration
 
 
@@ -95,7 +95,7 @@ def bench_bh(steps, warmup):
     walk_ms = tm["walk_ms"] / max(tm["count"], 1)
     walk_bytes = visits * WALK_RECORD_BYTES + n * WALK_PARTICLE_BYTES
     achieved = walk_bytes / (walk_ms * 1e-3) / 1e9
-    kernel = WALK_KERNELS[st_last["walk_mode"]]
+    kernel = WALK_KERNELS[st_last["walk_mode"]].format(w=st_last["walk_waves"])
     traffic = None
     fname = kernel.replace("<", "").replace(">", "")
     pmc = os.path.join(ROOT, "profiles", f"{fname}_diskbulge120k_pmc.json")

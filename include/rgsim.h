@@ -206,6 +206,7 @@ typedef struct {
     int64_t visits_max;     /* max nodes examined by one particle */
     int64_t nonfinite;      /* non-finite input coordinates seen */
     int64_t walk_mode;      /* FP32 walk variant used (0..6, see "walk_mode"); -1 FP64 */
+    int64_t walk_waves;     /* waves per group of 64 particles in it (0: one lane each) */
 } rg_bh_stats_t;
 
 /* Statistics of the last compute.  Synchronises that compute's stream. */

@@ -1557,7 +1557,8 @@ struct rg_bh {
     int final_buf = 0;      // ping-pong buffer holding the sorted result
     hipStream_t last_stream = nullptr;
     bool computed = false;
-    int walk_used = -1;  // walk variant of the last compute (-1 FP64)
+    int walk_used = -1;   // walk variant of the last compute (-1 FP64)
+    int walk_waves = 0;   // waves per group of 64 particles in that walk (0: per lane)
 
     uint64_t *keys[2] = {nullptr, nullptr};
     int *idx[2] = {nullptr, nullptr};
@@ -1827,6 +1828,7 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     const int wg = (n + wb - 1) / wb;
     const double eps2 = h->eps * h->eps;
     h->walk_used = -1;
+    h->walk_waves = 0;
     if (fp64) {
         k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, out, n, h->G, theta2, eps2, h->stats);
     } else {
@@ -1844,6 +1846,7 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
             const int waves = g_chunk_waves ? g_chunk_waves : (n < 150000 ? 8 : 4);
             const ChunkShape cs{g_chunk_dmin, shift, g_chunk_run};
             const int ng = (n + 63) / 64;
+            h->walk_waves = waves;
             if (waves == 8)
                 k_walk32_chunked<8><<<ng, 512, 0, st>>>(h->recA, h->recB, h->pre, order, out, n,
                                                         static_cast<float>(eps2), h->stats, cs);
@@ -1851,20 +1854,24 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
                 k_walk32_chunked<4><<<ng, 256, 0, st>>>(h->recA, h->recB, h->pre, order, out, n,
                                                         static_cast<float>(eps2), h->stats, cs);
         } else if (mode == 5) {
+            h->walk_waves = 4;
             k_walk32_wave_split<4><<<(n + 63) / 64, 256, 0, st>>>(h->recA, h->recB, h->pre,
                                                                   order, out, n,
                                                                   static_cast<float>(eps2),
                                                                   h->stats);
         } else if (mode == 4) {
+            h->walk_waves = 2;
             k_walk32_wave_split<2><<<(n + 127) / 128, 256, 0, st>>>(h->recA, h->recB, h->pre,
                                                                    order, out, n,
                                                                    static_cast<float>(eps2),
                                                                    h->stats);
         } else if (mode == 3) {
+            h->walk_waves = 1;
             k_walk32_wave2<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
                                                             out, n, static_cast<float>(eps2),
                                                             h->stats);
         } else if (mode == 2) {
+            h->walk_waves = 1;
             k_walk32_wave<<<(n + 255) / 256, 256, 0, st>>>(h->recA, h->recB, h->pre, order,
                                                            out, n, static_cast<float>(eps2),
                                                            h->stats);
@@ -1977,6 +1984,7 @@ extern "C" int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out)
     out->visits_max = static_cast<int64_t>(s[1]);
     out->nonfinite = static_cast<int64_t>(s[2]);
     out->walk_mode = h->walk_used;
+    out->walk_waves = h->walk_waves;
     RG_REQUIRE(s[3] == 0, RG_ERR_CAPACITY,
                "walk exceeded 2n node visits for %llu particles (corrupt tree)", s[3]);
     return RG_OK;

@@ -121,7 +121,7 @@ class BarnesHut:
         s = _lib.BHStats()
         _lib.check(_lib.load().rg_bh_stats(self._h, ctypes.byref(s)), "rg_bh_stats")
         return {"n": s.n, "visits_total": s.visits_total, "visits_max": s.visits_max,
-                "nonfinite": s.nonfinite, "walk_mode": s.walk_mode}
+                "nonfinite": s.nonfinite, "walk_mode": s.walk_mode, "walk_waves": s.walk_waves}
 
     def set_timing(self, enable: bool) -> None:
         """Record HIP events around each phase of every compute."""

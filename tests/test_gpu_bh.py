@@ -194,9 +194,12 @@ def test_fp32_walk_modes_agree(name):
         s = res[mode][1]
         assert s["walk_mode"] == mode
         assert s0["visits_total"] == s["visits_total"] and s0["visits_max"] == s["visits_max"], mode
-    # the automatic choice is the chunked wave walk
+    for mode, waves in ((2, 1), (3, 1), (4, 2), (5, 4)):
+        assert res[mode][1]["walk_waves"] == waves, mode
+    # the automatic choice is the chunked wave walk, 8 waves per group below 150k
     bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
-    assert bh.stats()["walk_mode"] == 6 and np.array_equal(acc, res[6][0])
+    st = bh.stats()
+    assert st["walk_mode"] == 6 and st["walk_waves"] == 8 and np.array_equal(acc, res[6][0])
     bh, _ = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp64")
     assert bh.stats()["walk_mode"] == -1
     scale = np.linalg.norm(a0, axis=1).max()
