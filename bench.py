@@ -30,6 +30,7 @@ for _p in (ROOT, os.path.join(ROOT, "real-astronomy-simulation_amd")):
 METRIC = ("direct Ginteractions/s & BH Mparticle-steps/s vs plain FP64 GPU, "
           "at stated force err")
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_GINST = 1024 * 2.4 / 4  # wave64 VALU instructions: 1024 SIMDs, 4 cycles each
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_INTERACTION = 20
@@ -104,6 +105,20 @@ def bench_bh(steps, warmup):
             rec = json.load(f)
         if rec.get("kernel") == kernel and rec.get("n") == n:
             traffic = rec.get("hbm_bytes_per_launch")
+    # the walk's actual limit is vector-instruction issue: its VALU wave
+    # instructions per launch (rocprofv3 SQ_INSTS_VALU, committed summary)
+    # over the measured launch time, against 1024 SIMDs x 2.4 GHz / 4 cycles
+    roofline_valu = None
+    sq = os.path.join(ROOT, "profiles", f"{fname}_diskbulge120k_sq.json")
+    if os.path.exists(sq):
+        with open(sq) as f:
+            rec = json.load(f)
+        if rec.get("kernel") == kernel and rec.get("n") == n:
+            rate = rec["valu_instr_per_launch"] / (walk_ms * 1e-3) / 1e9
+            roofline_valu = {"bound": "valu", "achieved": rate, "peak": VALU_PEAK_GINST,
+                             "unit": "G wave-instructions/s", "frac": rate / VALU_PEAK_GINST,
+                             "kernel": kernel,
+                             "valu_instr_per_launch": rec["valu_instr_per_launch"]}
     return {
         "n": n, "wall_s": wall, "ms_per_step": 1e3 * wall / steps,
         "mps": n * steps / wall / 1e6,
@@ -118,6 +133,7 @@ def bench_bh(steps, warmup):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "algorithmic_bytes_per_launch": walk_bytes,
                      "avg_launch_ms": walk_ms},
+        "roofline_valu": roofline_valu,
         "energy": None,
     }
 
@@ -571,8 +587,9 @@ def main():
                    "walk_precision": "fp32", "aggregation_precision": "fp64",
                    "parallelism": "single GPU"},
         "roofline": bh["roofline"],
+        "roofline_valu": bh["roofline_valu"],
         "cpu_baseline": cpu,
-        "bh": {k: v for k, v in bh.items() if k != "roofline"},
+        "bh": {k: v for k, v in bh.items() if k not in ("roofline", "roofline_valu")},
         "direct_plummer_262144": direct,
         "adjoint_plummer_16384": adjoint,
         "batch_512_systems": batch,
