@@ -105,6 +105,21 @@ def bench_bh(steps, warmup):
             rec = json.load(f)
         if rec.get("kernel") == kernel and rec.get("n") == n:
             traffic = rec.get("hbm_bytes_per_launch")
+    # the same workload with 30-bit keys (10 bits per axis, the option the
+    # workload text names; 4 sort passes instead of 8), 50 steps
+    e30 = Engine(softening=0.05, gravity_mode="bh", theta=0.5, precision="fp32",
+                 morton_bits=10)
+    e30.load_state(state)
+    for _ in range(5):
+        e30.step(dt)
+    torch.cuda.synchronize()
+    t30 = time.perf_counter()
+    for _ in range(50):
+        e30.step(dt)
+    torch.cuda.synchronize()
+    w30 = time.perf_counter() - t30
+    morton30 = {"morton_bits": 10, "steps": 50, "ms_per_step": 1e3 * w30 / 50,
+                "mps": n * 50 / w30 / 1e6}
     # the walk's actual limit is vector-instruction issue: its VALU wave
     # instructions per launch (rocprofv3 SQ_INSTS_VALU, committed summary)
     # over the measured launch time, against 1024 SIMDs x 2.4 GHz / 4 cycles
@@ -129,13 +144,33 @@ def bench_bh(steps, warmup):
                      "build": tm["build_ms"] / max(tm["count"], 1),
                      "aggregate": tm["aggregate_ms"] / max(tm["count"], 1),
                      "walk": walk_ms},
+        "phase_hbm": phase_rooflines(n, {"sort": tm["sort_ms"] / max(tm["count"], 1),
+                                         "build": tm["build_ms"] / max(tm["count"], 1),
+                                         "aggregate": tm["aggregate_ms"] / max(tm["count"], 1)}),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "algorithmic_bytes_per_launch": walk_bytes,
                      "avg_launch_ms": walk_ms},
         "roofline_valu": roofline_valu,
+        "morton_30bit": morton30,
         "energy": None,
     }
+
+
+def phase_rooflines(n, phase_ms, passes=8):
+    """HBM rooflines of the tree-build phases from their algorithmic bytes:
+    keys + sort 24 (bbox read) + 24 + 8 (Morton) + passes x 24 (sort) B per
+    particle; build 24 B per internal node + 4 B preorder slot per node;
+    aggregation 2 x 80 B per node."""
+    m = 2 * n - 1
+    nbytes = {"sort": n * (24 + 24 + 8 + passes * 24),
+              "build": 24 * (n - 1) + 4 * m,
+              "aggregate": 160 * m}
+    out = {}
+    for k, b in nbytes.items():
+        gbs = b / (phase_ms[k] * 1e-3) / 1e9
+        out[k] = {"algorithmic_bytes": b, "GB_per_s": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+    return out
 
 
 def bench_direct(n=262144, evals=3, kdk_steps=50):
@@ -332,9 +367,24 @@ def bench_bh_1m(steps=5):
     c = max(tm["count"], 1)
     walk_ms = tm["walk_ms"] / c
     walk_bytes = st["visits_total"] * WALK_RECORD_BYTES + n * WALK_PARTICLE_BYTES
+    phases = {"sort": tm["sort_ms"] / c, "build": tm["build_ms"] / c,
+              "aggregate": tm["aggregate_ms"] / c}
+    # the fused opening kick + drift (leapfrog.py:11-22): 120 B per particle
+    # (x and v read and written, a read), HBM-bound at this size
+    from rgsim.integrators import leapfrog
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    leapfrog.kick_drift(eng.pos, eng.vel, eng.acc, n, 0.0, 0.0)
+    e0.record()
+    for _ in range(20):
+        leapfrog.kick_drift(eng.pos, eng.vel, eng.acc, n, 0.0, 0.0)
+    e1.record()
+    torch.cuda.synchronize()
+    kd_ms = e0.elapsed_time(e1) / 20
+    kd_gbs = 120 * n / (kd_ms * 1e-3) / 1e9
     out = {"n": n, "ms_per_step": 1e3 * wall / steps, "mps": n * steps / wall / 1e6,
-           "phase_ms": {"sort": tm["sort_ms"] / c, "build": tm["build_ms"] / c,
-                        "aggregate": tm["aggregate_ms"] / c, "walk": walk_ms},
+           "phase_ms": dict(phases, walk=walk_ms),
+           "phase_hbm": phase_rooflines(n, phases),
+           "kick_drift": {"ms": kd_ms, "GB_per_s": kd_gbs, "frac_of_hbm_peak": kd_gbs / HBM_PEAK_GBS},
            "visits_per_particle": st["visits_total"] / n,
            "walk_GBs_algorithmic": walk_bytes / (walk_ms * 1e-3) / 1e9,
            "walk_frac_of_hbm_peak": walk_bytes / (walk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,

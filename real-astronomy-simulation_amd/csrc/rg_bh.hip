@@ -128,8 +128,7 @@ __global__ __launch_bounds__(kBoxBS) void k_bbox_final(const double *__restrict_
     if (ghist)  // the sort's global digit histograms and tile tickets start at zero
         for (int k = threadIdx.x; k < kMaxPasses * kBuckets; k += kBoxBS) ghist[k] = 0u;
     if (tickets && threadIdx.x < kMaxPasses) tickets[threadIdx.x] = 0;
-    // the tree's list counters (treelets, upper nodes, rank tickets, treelets
-    // in leaf order) start at zero
+    // the tree's counters (rank tickets at [2], treelets at [3]) start at zero
     if (counts && threadIdx.x < 4) counts[threadIdx.x] = 0;
     __shared__ double s[7][kBoxBS];
     const int t = threadIdx.x;
@@ -275,16 +274,14 @@ constexpr int kTreelet = 256;  // max leaves of a treelet (one workgroup, LDS)
 // subtrees with at most kTreelet leaves (a treelet root has at most kTreelet
 // leaves and its parent more); they partition the leaves.  The thread of
 // internal node i knows the leaf counts of i and of both children, so it
-// lists i as an upper node (more than kTreelet leaves) and each child that
-// is a treelet root; tl_of[first leaf of a treelet] = its root (tl_of was
-// cleared to -1 by k_morton).  Also clears the treelet-rank look-back words.
+// marks each child that is a treelet root: tl_of[first leaf of a treelet] =
+// its root (tl_of was cleared to -1 by k_morton).  The ranking pass then
+// lists the treelets in leaf order.  Also clears its look-back words.
 __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restrict__ left,
                         int *__restrict__ right, int *__restrict__ parent,
                         int *__restrict__ up, int2 *__restrict__ range,
-                        int *__restrict__ left_at_split, int *__restrict__ counts,
-                        int *__restrict__ tl_of, int *__restrict__ treelets,
-                        int *__restrict__ uppers, unsigned *__restrict__ rank_lb,
-                        int rank_tiles)
+                        int *__restrict__ left_at_split, int *__restrict__ tl_of,
+                        unsigned *__restrict__ rank_lb, int rank_tiles)
 {
     const int64_t nn = n;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rank_tiles;
@@ -329,17 +326,9 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
             up[nn] = -1;
         }
         if (hi - lo + 1 > kTreelet) {
-            uppers[atomicAdd(&counts[1], 1)] = self;
-            if (gamma - lo + 1 <= kTreelet) {
-                treelets[atomicAdd(&counts[0], 1)] = lc;
-                tl_of[lo] = lc;
-            }
-            if (hi - gamma <= kTreelet) {
-                treelets[atomicAdd(&counts[0], 1)] = rc;
-                tl_of[gamma + 1] = rc;
-            }
+            if (gamma - lo + 1 <= kTreelet) tl_of[lo] = lc;
+            if (hi - gamma <= kTreelet) tl_of[gamma + 1] = rc;
         } else if (i == 0) {  // the root is itself a treelet
-            treelets[atomicAdd(&counts[0], 1)] = self;
             tl_of[0] = self;
         }
     }
@@ -553,7 +542,7 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     const double *__restrict__ mass, int n, const int *__restrict__ left,
     const int *__restrict__ right, const int *__restrict__ parent,
     const int2 *__restrict__ range, const int *__restrict__ left_at_split,
-    const int *__restrict__ treelets, const int *__restrict__ counts, Agg *agg,
+    const int *__restrict__ tsorted, const int *__restrict__ counts, Agg *agg,
     WalkOut wo)
 {
     constexpr int C = kTreelet;
@@ -567,9 +556,9 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     __shared__ int s_L[C], s_R[C], s_par[C], s_pre[C];  // internal node lo + t
     __shared__ int2 s_rg[C];
     const int tid = threadIdx.x;
-    const int cnt = counts[0];
+    const int cnt = counts[3];  // treelets, in leaf order in tsorted
     for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
-        const int v = treelets[e];
+        const int v = tsorted[e];
         const int lo = v < n ? v : range[v - n].x;
         const int size = v < n ? 1 : range[v - n].y - lo + 1;
         s_arrive[tid] = 0;
@@ -687,14 +676,16 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
 // The nodes above the treelets: each is the sum of a run of consecutive
 // treelets in leaf order, [trank[lo], trank[hi + 1]), so every upper node is
 // summed on its own, with no ordering between nodes -- one workgroup per
-// node.  Thread t adds treelets A + t, A + t + 256, ... in order; the 256
+// node.  They are the T - 1 nodes whose split falls between two consecutive
+// treelets: the node split at leaf b - 1, b the first leaf of treelet j + 1,
+// is parent[left_at_split[b - 1]].  Thread t adds treelets A + t, A + t + 256, ... in order; the 256
 // partials are combined by a butterfly within each wave (both operands of
 // every add are the same on the two lanes, so all lanes agree) and the four
 // wave results in wave order: a fixed summation order.  Min/max are exact.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_agg_upper(
-    int n, const int *__restrict__ right, const int2 *__restrict__ range,
-    const int *__restrict__ left_at_split, const int *__restrict__ uppers,
+    int n, const int *__restrict__ right, const int *__restrict__ parent,
+    const int2 *__restrict__ range, const int *__restrict__ left_at_split,
     const int *__restrict__ counts, const int *__restrict__ trank,
     const int *__restrict__ tsorted, Agg *agg, WalkOut wo)
 {
@@ -702,10 +693,11 @@ __global__ __launch_bounds__(256) void k_agg_upper(
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int lane = t & 63;
-    const int nup = counts[1];
     const int T = counts[3];
-    for (int e = blockIdx.x; e < nup; e += gridDim.x) {
-        const int v = uppers[e];
+    for (int e = blockIdx.x; e < T - 1; e += gridDim.x) {
+        const int tr = tsorted[e + 1];
+        const int b = tr < n ? tr : range[tr - n].x;  // first leaf of treelet e + 1
+        const int v = parent[left_at_split[b - 1]];
         const int ii = v - n;
         const int2 rg = range[ii];
         const int A = trank[rg.x];
@@ -1568,7 +1560,7 @@ struct rg_bh {
     int *left = nullptr, *right = nullptr, *parent = nullptr;
     int2 *range = nullptr;
     int *left_at_split = nullptr;
-    int *up = nullptr, *pre = nullptr, *treelets = nullptr, *uppers = nullptr;
+    int *up = nullptr, *pre = nullptr;
     int *counts = nullptr, *tl_of = nullptr, *trank = nullptr, *tsorted = nullptr;
     unsigned *rank_lb = nullptr;  // treelet-rank look-back words, one per tile
     int rank_tiles = 0;
@@ -1704,8 +1696,6 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->left_at_split, 4 * (N - 1));
     ALLOC(h->parent, 4 * M);
     ALLOC(h->up, 4 * M);
-    ALLOC(h->treelets, 4 * N);
-    ALLOC(h->uppers, 4 * N);
     ALLOC(h->counts, 16);
     ALLOC(h->tl_of, 4 * N);
     ALLOC(h->trank, 4 * N);
@@ -1778,8 +1768,7 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
 
     const int g1 = min((n + 255) / 256, 16384);
     k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->up, h->range,
-                                h->left_at_split, h->counts, h->tl_of, h->treelets, h->uppers,
-                                h->rank_lb, h->rank_tiles);
+                                h->left_at_split, h->tl_of, h->rank_lb, h->rank_tiles);
     RG_CHECK_LAUNCH();
     const bool fp64 = h->precision == RG_FP64;
     {
@@ -1808,12 +1797,12 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     do {                                                                                     \
         k_agg_treelet<MODE><<<gt, kTreelet, 0, st>>>(order, pos, mass, n, h->left, h->right, \
                                                     h->parent, h->range, h->left_at_split,   \
-                                                    h->treelets, h->counts, h->agg,   \
+                                                    h->tsorted, h->counts, h->agg,    \
                                                     wo);                                     \
         RG_CHECK_LAUNCH();                                                                   \
-        k_agg_upper<MODE><<<gu, 256, 0, st>>>(n, h->right, h->range, h->left_at_split,       \
-                                              h->uppers, h->counts, h->trank, h->tsorted,    \
-                                              h->agg, wo);                                   \
+        k_agg_upper<MODE><<<gu, 256, 0, st>>>(n, h->right, h->parent, h->range,              \
+                                              h->left_at_split, h->counts, h->trank,         \
+                                              h->tsorted, h->agg, wo);                       \
     } while (0)
     if (fp64)
         RG_AGG_LAUNCH(0);
