@@ -1544,7 +1544,8 @@ struct rg_bh {
     int bits = 21;
     int precision = RG_FP64;
     int device = 0;
-    int nblk = 0;           // sort blocks
+    int nblk = 0;           // sort tiles
+    int sort_items = 0;     // keys per thread in a sort tile
     int passes = 0;         // radix passes
     int final_buf = 0;      // ping-pong buffer holding the sorted result
     hipStream_t last_stream = nullptr;
@@ -1669,7 +1670,8 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     }
     const size_t N = static_cast<size_t>(n);
     const size_t M = 2 * N - 1;
-    h->nblk = static_cast<int>((N + kSortTile - 1) / kSortTile);
+    h->sort_items = sort_items_for(static_cast<int64_t>(N));
+    h->nblk = sort_tiles(static_cast<int64_t>(N), h->sort_items);
     h->passes = (3 * morton_bits + kDigitBits - 1) / kDigitBits;
     int st = RG_OK;
 #define ALLOC(ptr, bytes)                                                  \
@@ -1751,12 +1753,11 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     // Stable LSD radix sort, kDigitBits-bit digits, one kernel per pass.
     int cur = 0;
     for (int p = 0; p < h->passes; ++p) {
-        k_onesweep<<<h->nblk, kSortBS, 0, st>>>(h->keys[cur], p == 0 ? nullptr : h->idx[cur],
-                                                h->keys[cur ^ 1], h->idx[cur ^ 1], n,
-                                                kDigitBits * p, h->ghist + kBuckets * p,
-                                                h->lookback + static_cast<size_t>(kBuckets) *
-                                                                  h->nblk * p,
-                                                h->tickets + p, h->stats);
+        launch_onesweep(h->sort_items, h->nblk, st, h->keys[cur], p == 0 ? nullptr : h->idx[cur],
+                        h->keys[cur ^ 1], h->idx[cur ^ 1], n, kDigitBits * p,
+                        h->ghist + kBuckets * p,
+                        h->lookback + static_cast<size_t>(kBuckets) * h->nblk * p,
+                        h->tickets + p, h->stats);
         RG_CHECK_LAUNCH();
         cur ^= 1;
     }

@@ -48,8 +48,9 @@ def case(name):
     raise KeyError(name)
 
 
-NAMES = ["mix2", "mix300", "mix5000", "galaxy120k", "dups", "same_point", "wide_box",
-         "min_radius", "one"]
+# mix600000 is above the key count where the sort switches to its larger tile
+NAMES = ["mix2", "mix300", "mix5000", "galaxy120k", "mix600000", "dups", "same_point",
+         "wide_box", "min_radius", "one"]
 
 
 @pytest.mark.parametrize("all_gas", [False, True])
