@@ -1377,6 +1377,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     __shared__ int s_b[kChunkMaxB + 1];
     __shared__ int s_nc;
     __shared__ int s_ctr;
+    __shared__ int s_done;
     __shared__ float s_part[kChunkMaxB][3][64];
     __shared__ int s_vis[64];
     const int wave = threadIdx.x >> 6;
@@ -1412,6 +1413,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             s_b[nbd + 1] = m;
             s_nc = nbd + 1;
             s_ctr = 0;
+            s_done = 0;
         }
         s_vis[lane] = 0;
     }
@@ -1515,8 +1517,20 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         if (visits) atomicAdd(&s_vis[lane], visits);
         wvis += visits;
     }
-    __syncthreads();
-    if (wave == 0) {
+    {
+        const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(wvis));
+        if (lane == 0) atomicAdd(&stats[0], tot);
+    }
+    // No block barrier: a wave that runs out of chunks leaves at once (its
+    // slot can take another block's wave), and the last wave to finish adds
+    // the chunk partials.  The release orders this wave's partial and visit
+    // writes before its count; the last wave's acquire sees every wave's.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    int done = 0;
+    if (lane == 0) done = atomicAdd(&s_done, 1);
+    done = __builtin_amdgcn_readfirstlane(__shfl(done, 0, 64));
+    if (done == NW - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         float ax = 0.f, ay = 0.f, az = 0.f;
         for (int c = 0; c < nc; ++c) {
             ax += s_part[c][0][lane];
@@ -1531,8 +1545,6 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         const int mx = wave_max_i32(s_vis[lane]);
         if (lane == 0) atomicMax(&stats[1], static_cast<unsigned long long>(mx));
     }
-    const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(wvis));
-    if (lane == 0) atomicAdd(&stats[0], tot);
 }
 
 }  // namespace

@@ -87,38 +87,50 @@ __global__ __launch_bounds__(kSortBS) void k_onesweep(
     constexpr int kSortTile = kSortBS * ITEMS;
     constexpr int DPT = kBuckets / kSortBS;    // digits owned per thread (adjacent)
     __shared__ unsigned s_base[kWaves][kBuckets];
-    __shared__ unsigned s_scan[kSortBS];
     __shared__ int s_tile;
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int lane = t & 63;
     if (t == 0) s_tile = atomicAdd(ticket_p, 1);
     for (int k = t; k < kWaves * kBuckets; k += kSortBS) (&s_base[0][0])[k] = 0u;
+    // Exclusive scan of this pass's global digit counts with no block
+    // barrier: every wave reads all of them, reduces the ranges of the
+    // threads in the waves before it and scans its own range.
     unsigned gh[DPT];
-    unsigned gsum = 0;
+    unsigned before = 0, gsum = 0;
 #pragma unroll
-    for (int q = 0; q < DPT; ++q) {
-        gh[q] = ghist_p[t * DPT + q];
-        gsum += gh[q];
+    for (int ww = 0; ww < kWaves; ++ww) {
+        unsigned v[DPT];
+        unsigned sum = 0;
+#pragma unroll
+        for (int q = 0; q < DPT; ++q) {
+            v[q] = ghist_p[(ww * 64 + lane) * DPT + q];
+            sum += v[q];
+        }
+        if (ww < w) {
+            before += __ockl_wfred_add_u32(sum);
+        } else if (ww == w) {
+            gsum = sum;
+#pragma unroll
+            for (int q = 0; q < DPT; ++q) gh[q] = v[q];
+        }
     }
-    s_scan[t] = gsum;
-    __syncthreads();
-    // exclusive scan of this pass's global digit counts
-    for (int off = 1; off < kSortBS; off <<= 1) {
-        const unsigned v = t >= off ? s_scan[t - off] : 0u;
-        __syncthreads();
-        s_scan[t] += v;
-        __syncthreads();
+    unsigned incl = gsum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
     }
     unsigned goff[DPT];
     {
-        unsigned run = s_scan[t] - gsum;
+        unsigned run = before + incl - gsum;
 #pragma unroll
         for (int q = 0; q < DPT; ++q) {
             goff[q] = run;
             run += gh[q];
         }
     }
+    __syncthreads();  // s_tile and the zeroed counts
     const int tile = s_tile;
 
     const int seg = tile * kSortTile + w * (kRounds * 64);
