@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -38,7 +39,7 @@ WALK_RECORD_BYTES = 32          # FP32 node record fetched per visit
 # FP32 walk variant (rg_bh_stats walk_mode) -> the kernel it launches
 WALK_KERNELS = {0: "k_walk32", 1: "k_walk32_pair", 2: "k_walk32_wave", 3: "k_walk32_wave2",
                 4: "k_walk32_wave_split<2>", 5: "k_walk32_wave_split<4>",
-                6: "k_walk32_chunked<{w}>", -1: "k_walk64"}
+                6: "k_walk32_chunked<{w}>", 7: "k_walk32_chunked<{w}, true>", -1: "k_walk64"}
 WALK_PARTICLE_BYTES = 16 + 4 + 24  # own leaf record + order index + FP64 acceleration
 
 
@@ -98,7 +99,7 @@ def bench_bh(steps, warmup):
     achieved = walk_bytes / (walk_ms * 1e-3) / 1e9
     kernel = WALK_KERNELS[st_last["walk_mode"]].format(w=st_last["walk_waves"])
     traffic = None
-    fname = kernel.replace("<", "").replace(">", "")
+    fname = re.sub(r"[^A-Za-z0-9_]", "", kernel.replace(", ", "_"))
     pmc = os.path.join(ROOT, "profiles", f"{fname}_diskbulge120k_pmc.json")
     if os.path.exists(pmc):  # rocprofv3 FETCH_SIZE / WRITE_SIZE of this kernel, copy-scaled
         with open(pmc) as f:

@@ -1368,7 +1368,7 @@ struct ChunkShape {
     int run;    // XCD run length (groups)
 };
 
-template <int NW>
+template <int NW, bool LW = false>
 __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     const float4 *__restrict__ recA, const int2 *__restrict__ recB, const int *__restrict__ pre,
     const int *__restrict__ order, AccOut out, int n, float eps2,
@@ -1378,8 +1378,14 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     __shared__ int s_nc;
     __shared__ int s_ctr;
     __shared__ int s_done;
-    __shared__ float s_part[kChunkMaxB][3][64];
+    // chunk partial sums [chunk][axis][lane], sized by the launch for the
+    // most chunks a group can have at this n
+    extern __shared__ float s_part_dyn[];
+    float(*s_part)[3][64] = reinterpret_cast<float(*)[3][64]>(s_part_dyn);
     __shared__ int s_vis[64];
+    // LW: each wave's window of (com, G m) records sits in LDS and a step
+    // reads its record with one broadcast load instead of four v_readlane
+    __shared__ float4 s_win[LW ? NW : 1][64];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int g = xcd_run_block(blockIdx.x, gridDim.x, cs.run);
@@ -1449,18 +1455,31 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 k0 = k;
                 const int slot = k0 + lane;
                 if (slot < m) {
-                    wa = recA[slot];
+                    if (LW)
+                        s_win[LW ? wave : 0][lane] = recA[slot];
+                    else
+                        wa = recA[slot];
                     wb = recB[slot];
                 }
+                // the window is read across lanes: keep the reads after the writes
+                if (LW) __builtin_amdgcn_wave_barrier();
             }
             const int l = k - k0;
             const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
             if (skip <= lo) {
                 if (next == k) next = skip;
             } else {
-                const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
-                const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
-                const float cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
+                float cx, cy, cz;
+                if (LW) {
+                    const float4 r = s_win[LW ? wave : 0][l];
+                    cx = r.x;
+                    cy = r.y;
+                    cz = r.z;
+                } else {
+                    cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
+                    cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
+                    cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
+                }
                 const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
                 if (next == k) {
                     const float dx = cx - px;
@@ -1481,15 +1500,33 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 k0 = k;
                 const int slot = k0 + lane;
                 if (slot < m) {
-                    wa = recA[slot];
+                    if (LW)
+                        s_win[LW ? wave : 0][lane] = recA[slot];
+                    else
+                        wa = recA[slot];
                     wb = recB[slot];
                 }
+                // the window is read across lanes: keep the reads after the writes
+                if (LW) __builtin_amdgcn_wave_barrier();
             }
             const int l = k - k0;
-            const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
-            const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
-            const float cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
-            const float cw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
+            float cx, cy, cz, cw;
+            if (LW) {
+                // volatile: one 16-byte broadcast load here, not split into
+                // a second one on the accept path
+                typedef float f4v __attribute__((ext_vector_type(4)));
+                typedef __attribute__((address_space(3))) volatile f4v lds_f4v;
+                const f4v r = *(lds_f4v *)(&s_win[LW ? wave : 0][l]);
+                cx = r.x;
+                cy = r.y;
+                cz = r.z;
+                cw = r.w;
+            } else {
+                cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
+                cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
+                cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
+                cw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
+            }
             const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
             const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
             if (next == k) {
@@ -1621,7 +1658,7 @@ int bh_mark(rg_bh *h, int phase, hipStream_t st)
 
 int rg::rg_bh_set_walk_mode_impl(int64_t v)
 {
-    RG_REQUIRE(v >= -1 && v <= 6, RG_ERR_ARG, "walk_mode must be -1 (auto) or 0..6");
+    RG_REQUIRE(v >= -1 && v <= 7, RG_ERR_ARG, "walk_mode must be -1 (auto) or 0..7");
     g_walk_mode = static_cast<int>(v);
     return RG_OK;
 }
@@ -1834,27 +1871,41 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     if (fp64) {
         k_walk64<<<wg, wb, 0, st>>>(h->rec64, order, out, n, h->G, theta2, eps2, h->stats);
     } else {
-        // The chunked walk measured fastest from 60k to 1M particles: its
-        // time is set by the longest wave, and chunking the far field
-        // shortens that wave the most.
-        const int mode = g_walk_mode >= 0 ? g_walk_mode : 6;
+        // The chunked walk with its record window in LDS measured fastest
+        // from 60k to 2M particles: its time is set by the longest wave,
+        // chunking the far field shortens that wave the most, and a step
+        // reads its record with one LDS broadcast load.
+        const int mode = g_walk_mode >= 0 ? g_walk_mode : 7;
         h->walk_used = mode;
-        if (mode == 6) {
-            // Measured on disk + bulge galaxies, 60k .. 1M particles: octave
-            // chunks (growth 2) until ~700k, then growth 4 (fewer ancestor
+        if (mode >= 6) {
+            // Measured on disk + bulge galaxies, 60k .. 2M particles: octave
+            // chunks (growth 2) below ~350k, then growth 4 (fewer ancestor
             // tests once the walk is throughput-bound); 8 waves per group only
-            // pay below ~150k, where the groups are too few to fill the GPU.
-            const int shift = g_chunk_shift ? g_chunk_shift : (n < 700000 ? 1 : 2);
-            const int waves = g_chunk_waves ? g_chunk_waves : (n < 150000 ? 8 : 4);
+            // pay below ~90k, where the groups are too few to fill the GPU.
+            const int shift = g_chunk_shift ? g_chunk_shift : (n < 350000 ? 1 : 2);
+            const int waves = g_chunk_waves ? g_chunk_waves : (n < 90000 ? 8 : 4);
             const ChunkShape cs{g_chunk_dmin, shift, g_chunk_run};
             const int ng = (n + 63) / 64;
             h->walk_waves = waves;
-            if (waves == 8)
-                k_walk32_chunked<8><<<ng, 512, 0, st>>>(h->recA, h->recB, h->pre, order, out, n,
-                                                        static_cast<float>(eps2), h->stats, cs);
+            // LDS for the chunk partial sums: the most chunks any group can
+            // have at this n (boundaries dmin * 2^(shift i) <= n - 1 per side)
+            int side = 0;
+            for (long long d = g_chunk_dmin; d <= n - 1 && side < kChunkSide; d <<= shift) ++side;
+            const int ncmax = 2 * side + 2;
+            const size_t smem = sizeof(float) * 3 * 64 * static_cast<size_t>(ncmax);
+            const float e2 = static_cast<float>(eps2);
+            if (mode == 7 && waves == 8)
+                k_walk32_chunked<8, true><<<ng, 512, smem, st>>>(h->recA, h->recB, h->pre, order,
+                                                                 out, n, e2, h->stats, cs);
+            else if (mode == 7)
+                k_walk32_chunked<4, true><<<ng, 256, smem, st>>>(h->recA, h->recB, h->pre, order,
+                                                                 out, n, e2, h->stats, cs);
+            else if (waves == 8)
+                k_walk32_chunked<8><<<ng, 512, smem, st>>>(h->recA, h->recB, h->pre, order, out,
+                                                           n, e2, h->stats, cs);
             else
-                k_walk32_chunked<4><<<ng, 256, 0, st>>>(h->recA, h->recB, h->pre, order, out, n,
-                                                        static_cast<float>(eps2), h->stats, cs);
+                k_walk32_chunked<4><<<ng, 256, smem, st>>>(h->recA, h->recB, h->pre, order, out,
+                                                           n, e2, h->stats, cs);
         } else if (mode == 5) {
             h->walk_waves = 4;
             k_walk32_wave_split<4><<<(n + 63) / 64, 256, 0, st>>>(h->recA, h->recB, h->pre,

@@ -182,7 +182,7 @@ def test_fp32_walk_modes_agree(name):
     from rgsim import _lib
     pos, mass = CASES[name]
     res = {}
-    for mode in (0, 1, 2, 3, 4, 5, 6):
+    for mode in (0, 1, 2, 3, 4, 5, 6, 7):
         _lib.set_param("walk_mode", mode)
         try:
             bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
@@ -190,16 +190,17 @@ def test_fp32_walk_modes_agree(name):
         finally:
             _lib.set_param("walk_mode", -1)
     (a0, s0) = res[0]
-    for mode in (1, 2, 3, 4, 5, 6):
+    for mode in (1, 2, 3, 4, 5, 6, 7):
         s = res[mode][1]
         assert s["walk_mode"] == mode
         assert s0["visits_total"] == s["visits_total"] and s0["visits_max"] == s["visits_max"], mode
     for mode, waves in ((2, 1), (3, 1), (4, 2), (5, 4)):
         assert res[mode][1]["walk_waves"] == waves, mode
-    # the automatic choice is the chunked wave walk, 8 waves per group below 150k
+    # the automatic choice is the chunked wave walk with its window in LDS,
+    # 8 waves per group below 90k
     bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
     st = bh.stats()
-    assert st["walk_mode"] == 6 and st["walk_waves"] == 8 and np.array_equal(acc, res[6][0])
+    assert st["walk_mode"] == 7 and st["walk_waves"] == 8 and np.array_equal(acc, res[7][0])
     bh, _ = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp64")
     assert bh.stats()["walk_mode"] == -1
     scale = np.linalg.norm(a0, axis=1).max()
@@ -207,6 +208,8 @@ def test_fp32_walk_modes_agree(name):
         assert np.abs(a0 - res[mode][0]).max() <= 1e-5 * scale, mode
     for mode in (2, 3):  # each lane's summation order unchanged
         assert np.array_equal(a0, res[mode][0]), mode
+    for mode in (7,):  # the same walk, records read from LDS
+        assert np.array_equal(res[6][0], res[mode][0]), mode
 
 
 def test_bh_1m_keys_order_topology_bitexact(orc):
