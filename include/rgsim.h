@@ -181,6 +181,16 @@ int rg_bh_compute(rg_bh *h, const double *pos, const double *mass, double *acc,
 int rg_bh_compute_kick(rg_bh *h, const double *pos, const double *mass, double *acc,
                        double *vel, double dt_half, void *stream);
 
+/* Engine.step's tree path (core/engine.py:108-116) in one call: kick(dt_half)
+ * and drift(dt) (core/integrators/leapfrog.py:11-22), the tree force into
+ * acc, and the closing kick(dt_half).  Bit-identical to rg_kick_drift
+ * followed by rg_bh_compute_kick; the opening kick + drift pass also boxes
+ * the drifted positions for the tree, which saves the separate box pass.
+ * acc holds the previous force on entry and the new one on return; pos, vel
+ * and acc must not overlap each other or mass. */
+int rg_bh_step(rg_bh *h, double *pos, double *vel, const double *mass, double *acc,
+               double dt_half, double dt, void *stream);
+
 /* BarnesHut.compute(pos, mass, acc_g) followed by add_acc(acc, acc_g, n), as
  * core/living_galaxy_engine.py:1017-1018 and core/gas_disk_engine.py:218-219
  * call them, with the add fused into the walk's store: acc[p] += a_p, the

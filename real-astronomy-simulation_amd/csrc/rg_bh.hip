@@ -66,74 +66,142 @@ struct alignas(16) WalkRec64 {
 };
 
 // --------------------------------------------------------------- bbox
-__global__ __launch_bounds__(kBoxBS) void k_bbox_partial(const double *__restrict__ pos, int n,
-                                                          double *__restrict__ part)
+// Block-wide min / max / sum of the 7 box values (lo xyz, hi xyz, non-finite
+// count): wave butterflies, then the 4 wave results combined in wave order
+// by every thread (so all threads hold the same result).  Min and max are
+// exact in any order; the count is a sum of integers held exactly.
+template <int BS = kBoxBS>
+__device__ __forceinline__ void block_box_reduce(double v[7], double (*s_w)[7])
 {
-    __shared__ double smin[3][kBoxBS];
-    __shared__ double smax[3][kBoxBS];
-    __shared__ double sbad[kBoxBS];
-    double lo[3] = {INFINITY, INFINITY, INFINITY};
-    double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    double bad = 0.0;
-    for (int i = blockIdx.x * kBoxBS + threadIdx.x; i < n; i += gridDim.x * kBoxBS) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const double v = pos[3 * i + a];
-            if (isfinite(v)) {
-                lo[a] = fmin(lo[a], v);
-                hi[a] = fmax(hi[a], v);
-            } else {
-                bad += 1.0;
-            }
+            v[a] = fmin(v[a], __shfl_xor(v[a], off, 64));
+            v[3 + a] = fmax(v[3 + a], __shfl_xor(v[3 + a], off, 64));
         }
+        v[6] += __shfl_xor(v[6], off, 64);
     }
-    const int t = threadIdx.x;
+    if (lane == 0)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        smin[a][t] = lo[a];
-        smax[a][t] = hi[a];
-    }
-    sbad[t] = bad;
+        for (int c = 0; c < 7; ++c) s_w[w][c] = v[c];
     __syncthreads();
-    for (int w = kBoxBS / 2; w > 0; w >>= 1) {
-        if (t < w) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                smin[a][t] = fmin(smin[a][t], smin[a][t + w]);
-                smax[a][t] = fmax(smax[a][t], smax[a][t + w]);
-            }
-            sbad[t] += sbad[t + w];
-        }
-        __syncthreads();
-    }
-    if (t == 0) {
-        const int nb = gridDim.x;
+    for (int c = 0; c < 7; ++c) v[c] = s_w[0][c];
+    for (int ww = 1; ww < BS / 64; ++ww) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            part[a * nb + blockIdx.x] = smin[a][0];
-            part[(3 + a) * nb + blockIdx.x] = smax[a][0];
+            v[a] = fmin(v[a], s_w[ww][a]);
+            v[3 + a] = fmax(v[3 + a], s_w[ww][3 + a]);
         }
-        part[6 * nb + blockIdx.x] = sbad[0];
+        v[6] += s_w[ww][6];
     }
 }
 
-// box = {lo_x, lo_y, lo_z, scale}; stats[0..1] reset, stats[2] = non-finite.
-__global__ __launch_bounds__(kBoxBS) void k_bbox_final(const double *__restrict__ part, int nb,
-                                                        int bits, double *__restrict__ box,
-                                                        unsigned long long *__restrict__ stats,
-                                                        unsigned *__restrict__ ghist,
-                                                        int *__restrict__ tickets,
-                                                        int *__restrict__ counts)
+// Resets, by block 0 of the first pass of a compute, for the Morton pass
+// and the sort and tree passes after it: the digit histograms, the sort's
+// tile tickets, the tree's counters (rank tickets at [2], treelets at [3])
+// and the visit and look-back-failure statistics (stats[0], [1], [3]; [2] is
+// written by the Morton pass).
+__device__ __forceinline__ void reset_counters(unsigned long long *__restrict__ stats,
+                                               unsigned *__restrict__ ghist,
+                                               int *__restrict__ tickets,
+                                               int *__restrict__ counts)
 {
-    if (ghist)  // the sort's global digit histograms and tile tickets start at zero
-        for (int k = threadIdx.x; k < kMaxPasses * kBuckets; k += kBoxBS) ghist[k] = 0u;
-    if (tickets && threadIdx.x < kMaxPasses) tickets[threadIdx.x] = 0;
-    // the tree's counters (rank tickets at [2], treelets at [3]) start at zero
-    if (counts && threadIdx.x < 4) counts[threadIdx.x] = 0;
-    __shared__ double s[7][kBoxBS];
     const int t = threadIdx.x;
+    if (ghist)
+        for (int k = t; k < kMaxPasses * kBuckets; k += blockDim.x) ghist[k] = 0u;
+    if (tickets && t < kMaxPasses) tickets[t] = 0;
+    if (counts && t < 4) counts[t] = 0;
+    if (stats && t == 0) {
+        stats[0] = 0ull;
+        stats[1] = 0ull;
+        stats[3] = 0ull;
+    }
+}
+
+// Per-block partial box of the finite coordinates and count of non-finite
+// ones (plus the counter resets in block 0).
+__global__ __launch_bounds__(kBoxBS) void k_bbox_partial(const double *__restrict__ pos, int n,
+                                                          double *__restrict__ part,
+                                                          unsigned long long *__restrict__ stats,
+                                                          unsigned *__restrict__ ghist,
+                                                          int *__restrict__ tickets,
+                                                          int *__restrict__ counts)
+{
+    __shared__ double s_w[kBoxBS / 64][7];
+    const int t = threadIdx.x;
+    if (blockIdx.x == 0) reset_counters(stats, ghist, tickets, counts);
     double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
-    for (int b = t; b < nb; b += kBoxBS) {
+    for (int i = blockIdx.x * kBoxBS + t; i < n; i += gridDim.x * kBoxBS) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double x = pos[3 * i + a];
+            if (isfinite(x)) {
+                v[a] = fmin(v[a], x);
+                v[3 + a] = fmax(v[3 + a], x);
+            } else {
+                v[6] += 1.0;
+            }
+        }
+    }
+    block_box_reduce(v, s_w);
+    if (t == 0) {
+        const int nb = gridDim.x;
+#pragma unroll
+        for (int c = 0; c < 7; ++c) part[c * nb + blockIdx.x] = v[c];
+    }
+}
+
+// The opening kick and drift of a KDK step (leapfrog.py:11-22: v += a h,
+// x += v dt, per component, the same two roundings as k_kick_drift) with the
+// partial box of the new positions, one thread per particle -- the drifted
+// positions are boxed while in registers instead of read back by a
+// separate pass.  Block 0 also resets the counters, as k_bbox_partial.
+constexpr int kKdBS = 1024;
+__global__ __launch_bounds__(kKdBS) void k_kick_drift_box(
+    double *__restrict__ pos, double *__restrict__ vel, const double *__restrict__ acc, int n,
+    double dt_half, double dt, double *__restrict__ part, unsigned long long *__restrict__ stats,
+    unsigned *__restrict__ ghist, int *__restrict__ tickets, int *__restrict__ counts)
+{
+    __shared__ double s_w[kKdBS / 64][7];
+    const int t = threadIdx.x;
+    if (blockIdx.x == 0) reset_counters(stats, ghist, tickets, counts);
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int i = blockIdx.x * kKdBS + t; i < n; i += gridDim.x * kKdBS) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int k = 3 * i + a;
+            const double vk = vel[k] + acc[k] * dt_half;
+            vel[k] = vk;
+            const double x = pos[k] + vk * dt;
+            pos[k] = x;
+            if (isfinite(x)) {
+                v[a] = fmin(v[a], x);
+                v[3 + a] = fmax(v[3 + a], x);
+            } else {
+                v[6] += 1.0;
+            }
+        }
+    }
+    block_box_reduce<kKdBS>(v, s_w);
+    if (t == 0) {
+        const int nb = gridDim.x;
+#pragma unroll
+        for (int c = 0; c < 7; ++c) part[c * nb + blockIdx.x] = v[c];
+    }
+}
+
+// The box from the nb partials, reduced by each block of the consumer (no
+// separate final launch): lo[3] and scale = (2^bits - 1) / (span *
+// 1.0000001), span = max over axes of (hi - lo), 1.0 if not positive
+// (barnes_hut.py:40-43).  Returns the non-finite count.
+__device__ double box_from_partials(const double *__restrict__ part, int nb, int bits,
+                                    double lo[3], double *scale, double (*s_w)[7])
+{
+    double v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0};
+    for (int b = threadIdx.x; b < nb; b += kBoxBS) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             v[a] = fmin(v[a], part[a * nb + b]);
@@ -141,41 +209,19 @@ __global__ __launch_bounds__(kBoxBS) void k_bbox_final(const double *__restrict_
         }
         v[6] += part[6 * nb + b];
     }
-#pragma unroll
-    for (int c = 0; c < 7; ++c) s[c][t] = v[c];
-    __syncthreads();
-    for (int w = kBoxBS / 2; w > 0; w >>= 1) {
-        if (t < w) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                s[a][t] = fmin(s[a][t], s[a][t + w]);
-                s[3 + a][t] = fmax(s[3 + a][t], s[3 + a][t + w]);
-            }
-            s[6][t] += s[6][t + w];
-        }
-        __syncthreads();
+    block_box_reduce(v, s_w);
+    double span = v[3] - v[0];
+    for (int a = 1; a < 3; ++a) {
+        const double sa = v[3 + a] - v[a];
+        if (sa > span) span = sa;
     }
-    if (t == 0) {
-        // barnes_hut.py:40-43: span = max over axes of (max - lo), 1.0 if not
-        // positive; scale = (2^bits - 1) / (span * 1.0000001).
-        double span = s[3][0] - s[0][0];
-        for (int a = 1; a < 3; ++a) {
-            const double sa = s[3 + a][0] - s[a][0];
-            if (sa > span) span = sa;
-        }
-        if (!(span > 0.0)) span = 1.0;
-        const double qmax = static_cast<double>((1ull << bits) - 1ull);
-        box[0] = s[0][0];
-        box[1] = s[1][0];
-        box[2] = s[2][0];
-        box[3] = qmax / (span * 1.0000001);
-        if (stats) {
-            stats[0] = 0ull;
-            stats[1] = 0ull;
-            stats[2] = static_cast<unsigned long long>(s[6][0]);
-            stats[3] = 0ull;
-        }
-    }
+    if (!(span > 0.0)) span = 1.0;
+    const double qmax = static_cast<double>((1ull << bits) - 1ull);
+    lo[0] = v[0];
+    lo[1] = v[1];
+    lo[2] = v[2];
+    *scale = qmax / (span * 1.0000001);
+    return v[6];
 }
 
 // ------------------------------------------------------------- morton
@@ -192,21 +238,32 @@ __device__ __forceinline__ uint64_t part1by2(uint64_t x)
 }
 
 // Keys, plus (when passes > 0) the global histogram of every radix digit
-// for the radix sort and the reset of the sort's look-back flags.
-__global__ __launch_bounds__(256) void k_morton(const double *__restrict__ pos, int n, int bits,
-                                                const double *__restrict__ box,
-                                                uint64_t *__restrict__ keys, int passes,
-                                                unsigned *__restrict__ ghist,
-                                                unsigned *__restrict__ lookback, int lb_words,
-                                                int *__restrict__ tl_of)
+// for the radix sort and the reset of the sort's look-back flags.  Every
+// block first reduces the bbox partials to the box; block 0 stores it and
+// the non-finite count (stats[2]).
+__global__ __launch_bounds__(kBoxBS) void k_morton(const double *__restrict__ pos, int n, int bits,
+                                                   const double *__restrict__ part, int nb,
+                                                   double *__restrict__ box,
+                                                   unsigned long long *__restrict__ stats,
+                                                   uint64_t *__restrict__ keys, int passes,
+                                                   unsigned *__restrict__ ghist,
+                                                   unsigned *__restrict__ lookback, int lb_words,
+                                                   int *__restrict__ tl_of)
 {
     __shared__ unsigned s_h[kMaxPasses * kBuckets];
-    for (int k = threadIdx.x; k < passes * kBuckets; k += blockDim.x) s_h[k] = 0u;
-    __syncthreads();
-    const double lo0 = box[0], lo1 = box[1], lo2 = box[2], scale = box[3];
+    __shared__ double s_w[kBoxBS / 64][7];
+    for (int k = threadIdx.x; k < passes * kBuckets; k += kBoxBS) s_h[k] = 0u;
+    double lo[3], scale;
+    const double bad = box_from_partials(part, nb, bits, lo, &scale, s_w);  // has a block barrier
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        box[0] = lo[0];
+        box[1] = lo[1];
+        box[2] = lo[2];
+        box[3] = scale;
+        if (stats) stats[2] = static_cast<unsigned long long>(bad);
+    }
     const double qmax = static_cast<double>((1ull << bits) - 1ull);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double lo[3] = {lo0, lo1, lo2};
+    for (int i = blockIdx.x * kBoxBS + threadIdx.x; i < n; i += gridDim.x * kBoxBS) {
         uint64_t key = 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -222,26 +279,40 @@ __global__ __launch_bounds__(256) void k_morton(const double *__restrict__ pos, 
             atomicAdd(&s_h[p * kBuckets + ((key >> (kDigitBits * p)) & kDigitMask)], 1u);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < passes * kBuckets; k += blockDim.x)
+    for (int k = threadIdx.x; k < passes * kBuckets; k += kBoxBS)
         if (s_h[k]) atomicAdd(&ghist[k], s_h[k]);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < lb_words; k += gridDim.x * blockDim.x)
+    for (int k = blockIdx.x * kBoxBS + threadIdx.x; k < lb_words; k += gridDim.x * kBoxBS)
         lookback[k] = 0u;
 }
+
+// The opening kick + drift fused into the box pass (rg_bh_step).
+struct KickDrift {
+    double *pos;
+    double *vel;
+    const double *acc;
+    double dt_half, dt;
+};
 
 int launch_bbox_morton(const double *pos, int n, int bits, double *part, double *box,
                        unsigned long long *stats, uint64_t *keys, int passes, unsigned *ghist,
                        int *tickets, unsigned *lookback, int lb_words, int *counts, int *tl_of,
-                       hipStream_t st)
+                       hipStream_t st, const KickDrift *kd = nullptr)
 {
-    const int nb = min((n + kBoxBS - 1) / kBoxBS, 1024);
-    k_bbox_partial<<<nb, kBoxBS, 0, st>>>(pos, n, part);
-    RG_CHECK_LAUNCH();
-    k_bbox_final<<<1, kBoxBS, 0, st>>>(part, nb, bits, box, stats, ghist, tickets, counts);
+    // at most 1024 partials (the part buffer's size) either way
+    int nb;
+    if (kd) {
+        nb = min((n + kKdBS - 1) / kKdBS, 1024);
+        k_kick_drift_box<<<nb, kKdBS, 0, st>>>(kd->pos, kd->vel, kd->acc, n, kd->dt_half, kd->dt,
+                                               part, stats, ghist, tickets, counts);
+    } else {
+        nb = min((n + kBoxBS - 1) / kBoxBS, 1024);
+        k_bbox_partial<<<nb, kBoxBS, 0, st>>>(pos, n, part, stats, ghist, tickets, counts);
+    }
     RG_CHECK_LAUNCH();
     // each block flushes passes x kBuckets global atomics: keep the grid modest
-    const int mg = min((n + 255) / 256, passes > 0 ? 256 : 2048);
-    k_morton<<<mg, 256, 0, st>>>(pos, n, bits, box, keys, passes, ghist, lookback, lb_words,
-                                 tl_of);
+    const int mg = min((n + kBoxBS - 1) / kBoxBS, passes > 0 ? 256 : 2048);
+    k_morton<<<mg, kBoxBS, 0, st>>>(pos, n, bits, part, nb, box, stats, keys, passes, ghist,
+                                    lookback, lb_words, tl_of);
     RG_CHECK_LAUNCH();
     return RG_OK;
 }
@@ -1383,9 +1454,12 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     extern __shared__ float s_part_dyn[];
     float(*s_part)[3][64] = reinterpret_cast<float(*)[3][64]>(s_part_dyn);
     __shared__ int s_vis[64];
-    // LW: each wave's window of (com, G m) records sits in LDS and a step
-    // reads its record with one broadcast load instead of four v_readlane
-    __shared__ float4 s_win[LW ? NW : 1][64];
+    // LW: each wave's window of (com, G m, accept threshold) sits in LDS,
+    // one 64-float row per component, and a step reads its record with
+    // broadcast loads (one address, row offsets) instead of five v_readlane;
+    // the skip slot stays in a register (v_readlane): it feeds the scalar
+    // choice of the next slot
+    __shared__ float s_win[LW ? NW : 1][5][64];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int g = xcd_run_block(blockIdx.x, gridDim.x, cs.run);
@@ -1442,6 +1516,9 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         const int lo = s_b[c], hi = s_b[c + 1];
         float ax = 0.f, ay = 0.f, az = 0.f;
         int visits = 0;
+        // A lane takes part at slot k when next <= k.  Accepting a node sets
+        // next to its skip slot; opening it leaves next alone (it stays <=
+        // k + 1), which saves a register write per step.
         int next = (active && lo < hi) ? 0 : m;
         int k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
         int k0 = -64;
@@ -1455,11 +1532,16 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 k0 = k;
                 const int slot = k0 + lane;
                 if (slot < m) {
-                    if (LW)
-                        s_win[LW ? wave : 0][lane] = recA[slot];
-                    else
-                        wa = recA[slot];
+                    wa = recA[slot];
                     wb = recB[slot];
+                    if (LW) {
+                        float *row = &s_win[LW ? wave : 0][0][lane];
+                        row[0] = wa.x;
+                        row[64] = wa.y;
+                        row[128] = wa.z;
+                        row[192] = wa.w;
+                        row[256] = __int_as_float(wb.x);
+                    }
                 }
                 // the window is read across lanes: keep the reads after the writes
                 if (LW) __builtin_amdgcn_wave_barrier();
@@ -1467,31 +1549,32 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             const int l = k - k0;
             const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
             if (skip <= lo) {
-                if (next == k) next = skip;
+                if (next <= k) next = skip;
             } else {
-                float cx, cy, cz;
+                float cx, cy, cz, crit;
                 if (LW) {
-                    const float4 r = s_win[LW ? wave : 0][l];
-                    cx = r.x;
-                    cy = r.y;
-                    cz = r.z;
+                    const float *row = &s_win[LW ? wave : 0][0][l];
+                    cx = row[0];
+                    cy = row[64];
+                    cz = row[128];
+                    crit = row[256];
                 } else {
                     cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
                     cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
                     cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
+                    crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
                 }
-                const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
-                if (next == k) {
+                if (next <= k) {
                     const float dx = cx - px;
                     const float dy = cy - py;
                     const float dz = cz - pz;
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-                    next = crit < r2 ? skip : k + 1;
+                    if (crit < r2) next = skip;  // opened: stays active (next <= k + 1)
                 }
             }
             // the smallest next slot: k + 1 if a lane opened k, else skip(k)
             // (every other lane waits at an ancestor's skip, >= skip(k))
-            k = __ballot(next == k + 1) ? k + 1 : skip;
+            k = __ballot(next <= k + 1) ? k + 1 : skip;
         }
         // The chunk's own slots: every node a lane reaches is tested, counted
         // and, when accepted, added.
@@ -1500,36 +1583,38 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 k0 = k;
                 const int slot = k0 + lane;
                 if (slot < m) {
-                    if (LW)
-                        s_win[LW ? wave : 0][lane] = recA[slot];
-                    else
-                        wa = recA[slot];
+                    wa = recA[slot];
                     wb = recB[slot];
+                    if (LW) {
+                        float *row = &s_win[LW ? wave : 0][0][lane];
+                        row[0] = wa.x;
+                        row[64] = wa.y;
+                        row[128] = wa.z;
+                        row[192] = wa.w;
+                        row[256] = __int_as_float(wb.x);
+                    }
                 }
                 // the window is read across lanes: keep the reads after the writes
                 if (LW) __builtin_amdgcn_wave_barrier();
             }
             const int l = k - k0;
-            float cx, cy, cz, cw;
+            float cx, cy, cz, cw, crit;
             if (LW) {
-                // volatile: one 16-byte broadcast load here, not split into
-                // a second one on the accept path
-                typedef float f4v __attribute__((ext_vector_type(4)));
-                typedef __attribute__((address_space(3))) volatile f4v lds_f4v;
-                const f4v r = *(lds_f4v *)(&s_win[LW ? wave : 0][l]);
-                cx = r.x;
-                cy = r.y;
-                cz = r.z;
-                cw = r.w;
+                const float *row = &s_win[LW ? wave : 0][0][l];
+                cx = row[0];
+                cy = row[64];
+                cz = row[128];
+                cw = row[192];
+                crit = row[256];
             } else {
                 cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
                 cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
                 cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
                 cw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
+                crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
             }
-            const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
             const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
-            if (next == k) {
+            if (next <= k) {
                 ++visits;
                 const float dx = cx - px;
                 const float dy = cy - py;
@@ -1542,11 +1627,9 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     ay = fmaf(s, dy, ay);
                     az = fmaf(s, dz, az);
                     next = skip;
-                } else {
-                    next = k + 1;
-                }
+                }  // opened: stays active (next <= k + 1)
             }
-            k = __ballot(next == k + 1) ? k + 1 : skip;
+            k = __ballot(next <= k + 1) ? k + 1 : skip;
         }
         s_part[c][0][lane] = ax;
         s_part[c][1][lane] = ay;
@@ -1771,7 +1854,7 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
 namespace {
 
 int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &out,
-               void *stream)
+               void *stream, const KickDrift *kd = nullptr)
 {
     RG_REQUIRE(h, RG_ERR_ARG, "null handle");
     RG_REQUIRE(pos && mass && out.acc, RG_ERR_ARG, "null pointer argument");
@@ -1796,7 +1879,7 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     const int lb_words = kBuckets * h->nblk * h->passes;
     rc = launch_bbox_morton(pos, n, h->bits, part, box, h->stats, h->keys[0], h->passes,
                             h->ghist, h->tickets, h->lookback, lb_words, h->counts, h->tl_of,
-                            st);
+                            st, kd);
     if (rc != RG_OK) return rc;
 
     // Stable LSD radix sort, kDigitBits-bit digits, one kernel per pass.
@@ -1965,6 +2048,17 @@ extern "C" int rg_bh_compute_kick(rg_bh *h, const double *pos, const double *mas
 {
     RG_REQUIRE(vel, RG_ERR_ARG, "null vel");
     return bh_compute(h, pos, mass, AccOut{acc, vel, dt_half, 0}, stream);
+}
+
+extern "C" int rg_bh_step(rg_bh *h, double *pos, double *vel, const double *mass, double *acc,
+                          double dt_half, double dt, void *stream)
+{
+    RG_REQUIRE(h, RG_ERR_ARG, "null handle");
+    RG_REQUIRE(pos && vel && mass && acc, RG_ERR_ARG, "null pointer argument");
+    const size_t b3 = 24 * static_cast<size_t>(h->n);
+    RG_REQUIRE(!rg_overlap(pos, b3, vel, b3), RG_ERR_ALIAS, "vel aliases pos");
+    const KickDrift kd{pos, vel, acc, dt_half, dt};
+    return bh_compute(h, pos, mass, AccOut{acc, vel, dt_half, 0}, stream, &kd);
 }
 
 extern "C" int rg_bh_set_timing(rg_bh *h, int enable)

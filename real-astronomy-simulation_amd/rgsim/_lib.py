@@ -70,6 +70,7 @@ SIGNATURES = {
     "rg_bh_compute": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "rg_bh_compute_add": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "rg_bh_compute_kick": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _vp]),
+    "rg_bh_step": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d, _vp]),
     "rg_bh_export_tree": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rg_bh_export_nodes": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rg_bh_stats": (_i, [_vp, ctypes.POINTER(BHStats)]),

@@ -9,9 +9,10 @@ float64 CUDA tensors and every kernel runs through the HIP C ABI:
 * Barnes-Hut is used iff ``gravity_mode == "bh" and n >= 256``
   (core/engine.py:75); otherwise ``_bh is None`` and the direct sum runs.
 * ``step(dt)`` is the reference's KDK order (core/engine.py:108-116): the
-  opening kick and the drift are one fused pass, and the closing kick is
-  fused into the force kernel (direct) or the tree walk's store (BH).  The
-  fusions are bit-identical to the separate reference kernels.
+  opening kick and the drift are one fused pass (which, for BH, also boxes
+  the positions for the tree), and the closing kick is fused into the force
+  kernel (direct) or the tree walk's store (BH).  The fusions are
+  bit-identical to the separate reference kernels.
 * ``energies()`` is FP64 (KE, PE, KE + PE) like core/engine.py:119-124.
 
 Extra keyword arguments: ``precision`` ("fp64" default = the reference's
@@ -117,13 +118,14 @@ class Engine:
     def step(self, dt: float) -> None:
         """One KDK step (core/engine.py:108-116)."""
         half = 0.5 * dt
-        leapfrog.kick_drift(self.pos, self.vel, self.acc, self.n, half, dt)
         if self._bh is not None:
-            # force, then the closing half-kick in the walk's store
-            self._bh.compute_kick(self.pos, self.mass, self.acc, self.vel, half)
+            # kick + drift (boxing the positions for the tree), force, and the
+            # closing half-kick in the walk's store
+            self._bh.step(self.pos, self.vel, self.mass, self.acc, half, dt)
             if self.check_finite:
                 self._check_bh()
         else:
+            leapfrog.kick_drift(self.pos, self.vel, self.acc, self.n, half, dt)
             gravity.compute_acc_direct_kick(self.pos, self.mass, self.acc, self.vel, self.n,
                                             self.G, self.softening ** 2, half, self.precision)
         self.time += dt

@@ -76,6 +76,19 @@ class BarnesHut:
                                                   float(dt_half), _lib.stream_ptr(pos)),
                    "rg_bh_compute_kick")
 
+    def step(self, pos, vel, mass, acc, dt_half, dt) -> None:
+        """One KDK step of the tree path (core/engine.py:108-116): kick, drift,
+        compute(pos, mass, acc), kick -- bit-identical to
+        leapfrog.kick_drift followed by compute_kick, with the opening
+        kick + drift pass also boxing the positions for the tree.  acc holds
+        the previous force on entry and the new one after."""
+        n = self.n
+        vec3(pos, n, "pos"), vec3(vel, n, "vel"), vec1(mass, n, "mass"), vec3(acc, n, "acc")
+        _lib.check(_lib.load().rg_bh_step(self._h, pos.data_ptr(), vel.data_ptr(),
+                                          mass.data_ptr(), acc.data_ptr(), float(dt_half),
+                                          float(dt), _lib.stream_ptr(pos)),
+                   "rg_bh_step")
+
     def compute_add(self, pos, mass, acc) -> None:
         """acc += the treecode acceleration: compute(pos, mass, acc_g) then
         acc += acc_g, as living_galaxy_engine.py:1017-1018 and

@@ -147,6 +147,54 @@ def test_bh_compute_add_is_compute_then_add(precision, n):
     assert st["n"] == n and st["visits_total"] > 0
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("n", [256, 6000])
+def test_bh_step_is_kick_drift_then_compute_kick(precision, n):
+    """rg_bh_step (the Engine's tree step, core/engine.py:108-116) equals
+    leapfrog.kick_drift followed by compute_kick bit for bit over several
+    steps: positions, velocities, forces and the walk statistics, including
+    the non-finite count of the drifted positions."""
+    from rgsim.integrators import leapfrog
+    from rgsim.solvers.barnes_hut import BarnesHut
+    print(f"bh step {precision} n={n}", flush=True)
+    pos, mass, vel = gas_like(n, 18)
+    vel *= 1e-3
+    m = dev(mass)
+    runs = []
+    for fused in (False, True):
+        bh = BarnesHut(n, theta=0.6, softening=0.05, precision=precision)
+        p, v = dev(pos), dev(vel)
+        a = torch.empty_like(p)
+        bh.compute(p, m, a)
+        stats = []
+        for _ in range(3):
+            if fused:
+                bh.step(p, v, m, a, 5e-5, 1e-4)
+            else:
+                leapfrog.kick_drift(p, v, a, n, 5e-5, 1e-4)
+                bh.compute_kick(p, m, a, v, 5e-5)
+            st = bh.stats()
+            stats.append((st["visits_total"], st["visits_max"], st["nonfinite"]))
+        runs.append((p.cpu().numpy(), v.cpu().numpy(), a.cpu().numpy(), stats))
+    for x, y in zip(runs[0][:3], runs[1][:3]):
+        assert np.array_equal(x, y)
+    assert runs[0][3] == runs[1][3]
+    # a non-finite velocity drifts into a non-finite position: both paths count it
+    counts = []
+    for fused in (False, True):
+        bh = BarnesHut(n, theta=0.6, softening=0.05, precision=precision)
+        p, v = dev(pos), dev(vel)
+        a = torch.zeros_like(p)
+        v[3, 1] = float("inf")
+        if fused:
+            bh.step(p, v, m, a, 5e-5, 1e-4)
+        else:
+            leapfrog.kick_drift(p, v, a, n, 5e-5, 1e-4)
+            bh.compute_kick(p, m, a, v, 5e-5)
+        counts.append(bh.stats()["nonfinite"])
+    assert counts[0] == counts[1] == 1
+
+
 def test_engine_entry_points_reject_bad_arguments():
     from rgsim import _lib
     from rgsim.solvers import gravity
