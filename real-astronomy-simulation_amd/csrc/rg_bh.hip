@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kBoxBS) void k_bbox_partial(const double *__restric
 // partial box of the new positions, one thread per particle -- the drifted
 // positions are boxed while in registers instead of read back by a
 // separate pass.  Block 0 also resets the counters, as k_bbox_partial.
-constexpr int kKdBS = 1024;
+constexpr int kKdBS = 256;  // 1024-thread blocks left half the CUs idle at 120k
 __global__ __launch_bounds__(kKdBS) void k_kick_drift_box(
     double *__restrict__ pos, double *__restrict__ vel, const double *__restrict__ acc, int n,
     double dt_half, double dt, double *__restrict__ part, unsigned long long *__restrict__ stats,

@@ -93,12 +93,39 @@ class Engine:
         self._compute_acc()
 
     def to_state(self) -> State:
-        """Download the state (core/engine.py:83-98)."""
+        """Download the state (core/engine.py:83-98).  ``meta`` also records
+        this engine's solver settings (``engine_*`` keys), so that
+        ``Engine.from_state`` resumes with the same precision and tree."""
+        meta = dict(self.meta)
+        meta.update(self.settings())
         return State(pos=self.pos.cpu().numpy(), vel=self.vel.cpu().numpy(),
                      mass=self.mass.cpu().numpy(), ptype=self._ptype, ids=self._ids,
                      time=self.time, step=self.step_count, seed=self.seed, age=self._age,
                      temperature=self._temperature, metallicity=self._metallicity,
-                     meta=dict(self.meta))
+                     meta=meta)
+
+    def settings(self) -> dict:
+        """The constructor arguments, as snapshot meta entries."""
+        return {"engine_softening": self.softening, "engine_gravity_mode": self.gravity_mode,
+                "engine_theta": self.theta, "engine_precision": self.precision,
+                "engine_G": self.G, "engine_morton_bits": self.morton_bits}
+
+    @classmethod
+    def from_state(cls, state: State, **overrides) -> "Engine":
+        """An engine built with the settings a snapshot recorded (keyword
+        arguments override them), loaded with the snapshot.  A snapshot
+        without them gets the constructor defaults."""
+        m = state.meta
+        kw = {"softening": float(m.get("engine_softening", 0.05)),
+              "gravity_mode": str(m.get("engine_gravity_mode", "direct")),
+              "theta": float(m.get("engine_theta", 0.6)),
+              "precision": str(m.get("engine_precision", "fp64")),
+              "G": float(m.get("engine_G", G_GAL)),
+              "morton_bits": int(m.get("engine_morton_bits", 21))}
+        kw.update(overrides)
+        eng = cls(**kw)
+        eng.load_state(state)
+        return eng
 
     # ---------------------------------------------------------- stepping
     def _compute_acc(self) -> None:

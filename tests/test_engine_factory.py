@@ -1,0 +1,54 @@
+"""Host-side checks of the engine factory (core/simulation.py:20-33 with the
+tree solver wired in) and of the snapshot containers; no GPU needed."""
+import numpy as np
+import pytest
+
+
+def test_make_engine_maps_solver_names():
+    from rgsim.simulation import make_engine
+    e = make_engine("direct", 0.15)
+    assert e.gravity_mode == "direct" and e.softening == 0.15 and e._bh is None
+    for name in ("tree", "bh"):
+        e = make_engine(name, 0.05, {"theta": 0.5}, precision="fp32")
+        assert e.gravity_mode == "bh" and e.theta == 0.5 and e.precision == "fp32"
+    assert make_engine("tree", 0.05).theta == 0.6
+    with pytest.raises(NotImplementedError):
+        make_engine("test_particle", 0.0)
+    with pytest.raises(ValueError):
+        make_engine("sph", 0.1)
+
+
+def test_engine_settings_round_trip_through_npz(tmp_path):
+    """The engine_* meta entries survive save_npz/load_npz, and from_state's
+    arguments come from them (checked without loading onto a GPU)."""
+    from rgsim.engine import Engine
+    from rgsim.state import State
+    e = Engine(softening=0.07, gravity_mode="bh", theta=0.45, precision="fp32c",
+               morton_bits=10)
+    rng = np.random.default_rng(3)
+    s = State(pos=rng.normal(size=(9, 3)), vel=rng.normal(size=(9, 3)), mass=np.ones(9),
+              ptype=np.zeros(9), ids=np.arange(9), time=1.5, step=7, seed=4,
+              age=rng.random(9), meta={"level": "medium", **e.settings()})
+    path = str(tmp_path / "s.npz")
+    s.save_npz(path)
+    t = State.load_npz(path)
+    assert t.meta == s.meta and t.step == 7 and t.time == 1.5 and t.seed == 4
+    for name in ("pos", "vel", "mass", "ptype", "ids", "age"):
+        assert np.array_equal(getattr(t, name), getattr(s, name))
+    assert t.temperature is None
+
+
+def test_hdf5_snapshot_needs_h5py(tmp_path):
+    """save/load keep the reference's HDF5 layout and import h5py on use."""
+    from rgsim.state import State
+    s = State(pos=np.zeros((2, 3)), vel=np.zeros((2, 3)), mass=np.ones(2), ptype=np.zeros(2),
+              ids=np.arange(2), meta={"engine_precision": "fp32c"})
+    try:
+        import h5py  # noqa: F401
+    except ModuleNotFoundError:
+        with pytest.raises(ModuleNotFoundError):
+            s.save(str(tmp_path / "s.h5"))
+        return
+    s.save(str(tmp_path / "s.h5"))
+    t = State.load(str(tmp_path / "s.h5"))
+    assert t.meta == s.meta and np.array_equal(t.mass, s.mass)

@@ -117,3 +117,35 @@ def test_resume_is_bitexact(tmp_path):
     assert np.array_equal(a.to_state().pos, c.to_state().pos)
     assert np.array_equal(a.to_state().vel, c.to_state().vel)
     assert c.step_count == 40
+
+
+@pytest.mark.parametrize("mode,precision", [("bh", "fp32"), ("bh", "fp64"), ("direct", "fp32c")])
+def test_resume_from_snapshot_keeps_solver(tmp_path, mode, precision):
+    """A snapshot records the engine's settings; Engine.from_state resumes
+    with the same solver and precision, and the continued run equals the
+    uninterrupted one bit for bit (the snapshot-and-resume row of the tree
+    and reduced-precision paths)."""
+    from rgsim.engine import Engine
+    from rgsim.ic import make_disk_bulge
+    from rgsim.state import State
+    print(f"resume {mode} {precision}", flush=True)
+    d = make_disk_bulge(n_disk=2500, n_bulge=500, seed=5)
+    kw = dict(softening=0.05, gravity_mode=mode, theta=0.5, precision=precision)
+    a = Engine(**kw)
+    a.load_state(d)
+    for _ in range(12):
+        a.step(1e-4)
+    b = Engine(**kw)
+    b.load_state(d)
+    for _ in range(6):
+        b.step(1e-4)
+    path = str(tmp_path / "snap.npz")
+    b.to_state().save_npz(path)
+    c = Engine.from_state(State.load_npz(path))
+    assert (c.gravity_mode, c.precision, c.theta) == (mode, precision, 0.5)
+    assert (c._bh is not None) == (mode == "bh")
+    for _ in range(6):
+        c.step(1e-4)
+    sa, sc = a.to_state(), c.to_state()
+    assert np.array_equal(sa.pos, sc.pos) and np.array_equal(sa.vel, sc.vel)
+    assert c.step_count == 12 and sc.meta["engine_precision"] == precision
