@@ -106,21 +106,6 @@ def bench_bh(steps, warmup):
             rec = json.load(f)
         if rec.get("kernel") == kernel and rec.get("n") == n:
             traffic = rec.get("hbm_bytes_per_launch")
-    # the same workload with 30-bit keys (10 bits per axis, the option the
-    # workload text names; 4 sort passes instead of 8), 50 steps
-    e30 = Engine(softening=0.05, gravity_mode="bh", theta=0.5, precision="fp32",
-                 morton_bits=10)
-    e30.load_state(state)
-    for _ in range(5):
-        e30.step(dt)
-    torch.cuda.synchronize()
-    t30 = time.perf_counter()
-    for _ in range(50):
-        e30.step(dt)
-    torch.cuda.synchronize()
-    w30 = time.perf_counter() - t30
-    morton30 = {"morton_bits": 10, "steps": 50, "ms_per_step": 1e3 * w30 / 50,
-                "mps": n * 50 / w30 / 1e6}
     # the walk's actual limit is vector-instruction issue: its VALU wave
     # instructions per launch (rocprofv3 SQ_INSTS_VALU, committed summary)
     # over the measured launch time, against 1024 SIMDs x 2.4 GHz / 4 cycles
@@ -153,9 +138,32 @@ def bench_bh(steps, warmup):
                      "kernel": kernel, "algorithmic_bytes_per_launch": walk_bytes,
                      "avg_launch_ms": walk_ms},
         "roofline_valu": roofline_valu,
-        "morton_30bit": morton30,
         "energy": None,
     }
+
+
+def bench_bh30(state, dt=1e-4, steps=50):
+    """The BH workload with 30-bit keys (10 bits per axis, the option the
+    workload text names; 4 sort passes instead of 8), 50 steps.  Its own
+    section, so that a profile of the bh section holds only 63-bit walks."""
+    import torch
+
+    from rgsim.engine import Engine
+
+    n = state.n
+    e30 = Engine(softening=0.05, gravity_mode="bh", theta=0.5, precision="fp32",
+                 morton_bits=10)
+    e30.load_state(state)
+    for _ in range(5):
+        e30.step(dt)
+    torch.cuda.synchronize()
+    t30 = time.perf_counter()
+    for _ in range(steps):
+        e30.step(dt)
+    torch.cuda.synchronize()
+    w30 = time.perf_counter() - t30
+    return {"morton_bits": 10, "steps": steps, "ms_per_step": 1e3 * w30 / steps,
+            "mps": n * steps / w30 / 1e6}
 
 
 def phase_rooflines(n, phase_ms, passes=8):
@@ -213,6 +221,25 @@ def bench_direct(n=262144, evals=3, kdk_steps=50):
         log(f"direct {prec}: {ms:.2f} ms/eval, {gint:.1f} Gint/s")
     out["speedup_fp32c_vs_fp64"] = out["fp64"]["ms_per_eval"] / out["fp32c"]["ms_per_eval"]
     out["speedup_fp32_vs_fp64"] = out["fp64"]["ms_per_eval"] / out["fp32"]["ms_per_eval"]
+    # the FP32/FP32C sums above run split over j (32 slices at this n, added
+    # in order in FP64); the same FP32C sum with one slice, for comparison
+    from rgsim import _lib
+    _lib.set_param("direct_split", 1)
+    try:
+        acc = torch.empty_like(pos)
+        gravity.compute_acc_direct(pos, mass, acc, n, 1.0, eps2, "fp32c")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(evals):
+            gravity.compute_acc_direct(pos, mass, acc, n, 1.0, eps2, "fp32c")
+        e1.record()
+        torch.cuda.synchronize()
+    finally:
+        _lib.set_param("direct_split", 0)
+    ms1 = e0.elapsed_time(e1) / evals
+    out["fp32c_unsplit"] = {"ms_per_eval": ms1, "Gint_per_s": n * n / (ms1 * 1e-3) / 1e9,
+                            "force_err_vs_fp64": rel_err(acc.cpu().numpy(), ref)}
     # precision form: 50 KDK steps dt 5e-4, FP64 energy and momentum
     eng = Engine(softening=0.005, precision="fp32c", G=1.0)
     eng.load_state(s)
@@ -562,8 +589,9 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the BH drop-in workload (no direct/adjoint/batch/1M sections)")
-    ap.add_argument("--sections", default="bh,direct,adjoint,batch,bh1m,engines,grid",
-                    help="comma-separated subset of bh,direct,adjoint,batch,bh1m,engines,grid")
+    ap.add_argument("--sections", default="bh,bh30,direct,adjoint,batch,bh1m,engines,grid",
+                    help="comma-separated subset of bh,bh30,direct,adjoint,batch,bh1m,"
+                         "engines,grid")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -581,7 +609,7 @@ def main():
 
     sections = set(args.sections.split(","))
     if args.no_extra:
-        sections &= {"bh"}
+        sections &= {"bh", "bh30"}
     if args.no_direct:
         sections.discard("direct")
     bh = direct = adjoint = batch = bh1m = engines = grid = None
@@ -589,6 +617,11 @@ def main():
         log("bench: BH disk+bulge N=120000 ...")
         bh = bench_bh(args.steps, args.warmup)
         log(f"bench: BH {bh['mps']:.2f} Mparticle-steps/s, {bh['ms_per_step']:.3f} ms/step")
+        if "bh30" in sections:
+            from rgsim.ic import make_disk_bulge
+            bh["morton_30bit"] = bench_bh30(make_disk_bulge(n_disk=100000, n_bulge=20000,
+                                                            seed=1))
+            log(f"bench: BH 30-bit keys {bh['morton_30bit']['ms_per_step']:.3f} ms/step")
     if "direct" in sections:
         log("bench: direct Plummer N=262144 ...")
         direct = bench_direct()

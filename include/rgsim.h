@@ -10,9 +10,11 @@
  *    (n, 3) row-major FP64, masses (n,) FP64.  The caller owns them.
  *  - `stream` is a hipStream_t (NULL = the default stream).  Calls are
  *    stream-ordered and asynchronous unless stated otherwise.
- *  - Scratch memory: a BH handle owns its scratch.  rg_energy and
- *    rg_morton_codes use a per-device library buffer, so two of those calls
- *    that overlap on one device (different streams, host threads) race.
+ *  - Scratch memory: a BH handle owns its scratch.  rg_energy,
+ *    rg_morton_codes, and the FP32/FP32C direct sums and rg_direct_vjp when
+ *    they split j (from n = 257 up by default; at most 2^23 x 24 B) use a
+ *    per-device library buffer, so two of those calls that overlap on one
+ *    device (different streams, host threads) race.
  *  - `precision`: RG_FP64 reproduces the reference arithmetic; RG_FP32 is
  *    plain FP32 pair arithmetic; RG_FP32C is FP32 pair arithmetic on
  *    double-float (hi+lo) positions with compensated accumulation.
@@ -53,6 +55,14 @@ const char *rg_last_error(void);
 /* Launch-shape tunables (no reference counterpart; used by the autotuner):
  *   "direct_ipt"  i-particles per lane in the direct kernels: 0 (heuristic),
  *                 1, 2 or 4
+ *   "direct_split" j-slices per i-block of the FP32/FP32C direct sums and of
+ *                 rg_direct_vjp: 0 (default: 32, fewer when 32 x n would
+ *                 pass 2^23, i.e. above n = 262,144) or 1..256, and never
+ *                 more than n/256 (a slice holds at least one 256-particle
+ *                 tile).  Slice sums are added in slice order in FP64, so
+ *                 results are deterministic for a given n and setting; 1
+ *                 gives the unsplit sum.  FP64 direct sums are never split
+ *                 (they keep the reference's j order).
  *   "walk_block"  threads per block of the BH walk: 64, 128, 256 or 512
  *   "walk_mode"   FP32 BH walk: -1 auto (default: 7), 0 one lane per
  *                 particle, 1 two lanes per particle, 2 one wave walking
@@ -279,7 +289,9 @@ int rg_grid_destroy(rg_grid *g);
  * s2 = |d|^2 + eps2 and T = I/s^3 - 3 d d^T / s^5:
  *   g_pos[k]  = G sum_j T_kj (m_k lam_j - m_j lam_k)
  *   g_mass[k] = -G sum_j lam_j . d / s^3
- * Overwrites g_pos (n,3) and g_mass (n).  precision RG_FP64 or RG_FP32C. */
+ * Overwrites g_pos (n,3) and g_mass (n).  precision RG_FP64 or RG_FP32C.
+ * The j range is split as in the FP32 direct sums ("direct_split"), in
+ * either precision, and the slice sums are added in slice order in FP64. */
 int rg_direct_vjp(const double *pos, const double *mass, const double *lam,
                   double *g_pos, double *g_mass, int64_t n, double G, double eps2,
                   int precision, void *stream);
@@ -302,7 +314,8 @@ int rg_kdk_adjoint_update(double *xbar, double *vbar, double *mbar, const double
  * (Ntot,3)/(Ntot) FP64 arrays; G, eps2 (> 0) and dt are per system
  * (HOST arrays of nsys, copied).  precision RG_FP64 or RG_FP32C; each
  * system's accelerations are bitwise those of rg_direct_acc on that system
- * alone.  Not thread-safe per handle. */
+ * alone, run unsplit ("direct_split" 1; FP64 sums always are).  Not
+ * thread-safe per handle. */
 typedef struct rg_batch rg_batch;
 int rg_batch_create(int64_t nsys, const int64_t *offsets, const double *G,
                     const double *eps2, const double *dt, int precision, rg_batch **out);

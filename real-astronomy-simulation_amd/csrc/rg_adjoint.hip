@@ -10,6 +10,8 @@
 //   g_m[k] = -G sum_j lam_j . d_kj / s^3
 // (the j == k terms vanish).  One fused pass per k over all j: the same LDS
 // j-tiling as the forward kernel, with (x, m, lam) staged per j.
+#include <algorithm>
+
 #include "rg_internal.h"
 
 namespace {
@@ -30,12 +32,34 @@ __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
     e = (a - (s - bb)) + (b - bb);
 }
 
+// Particle i's sums over the block's j-slice (blockIdx.y): scaled into the
+// outputs when there is one slice (part == nullptr), else stored as
+// part[(y*4 + c)*n + i] for k_vjp_reduce.
+__device__ __forceinline__ void finish_vjp(int i, double gx, double gy, double gz, double gm,
+                                           double *__restrict__ gpos,
+                                           double *__restrict__ gmass, int n, double G,
+                                           double *__restrict__ part)
+{
+    if (part) {
+        double *p = part + static_cast<size_t>(blockIdx.y) * 4 * n;
+        p[i] = gx;
+        p[n + i] = gy;
+        p[2 * static_cast<size_t>(n) + i] = gz;
+        p[3 * static_cast<size_t>(n) + i] = gm;
+    } else {
+        gpos[3 * i] = G * gx;
+        gpos[3 * i + 1] = G * gy;
+        gpos[3 * i + 2] = G * gz;
+        gmass[i] = -G * gm;
+    }
+}
+
 __global__ __launch_bounds__(kBS) void k_vjp64(const double *__restrict__ pos,
                                                const double *__restrict__ mass,
                                                const double *__restrict__ lam,
                                                double *__restrict__ gpos,
                                                double *__restrict__ gmass, int n, double G,
-                                               double eps2)
+                                               double eps2, int jlen, double *__restrict__ part)
 {
     __shared__ double4 sx[kBS];
     __shared__ double4 sl[kBS];
@@ -48,14 +72,16 @@ __global__ __launch_bounds__(kBS) void k_vjp64(const double *__restrict__ pos,
     const double lxi = ok ? lam[3 * i] : 0.0, lyi = ok ? lam[3 * i + 1] : 0.0,
                  lzi = ok ? lam[3 * i + 2] : 0.0;
     double gx = 0.0, gy = 0.0, gz = 0.0, gm = 0.0;
-    for (int t0 = 0; t0 < n; t0 += kBS) {
+    const int j0 = blockIdx.y * jlen;
+    const int j1 = min(n, j0 + jlen);
+    for (int t0 = j0; t0 < j1; t0 += kBS) {
         const int j = t0 + tid;
-        if (j < n) {
+        if (j < j1) {
             sx[tid] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], mass[j]);
             sl[tid] = make_double4(lam[3 * j], lam[3 * j + 1], lam[3 * j + 2], 0.0);
         }
         __syncthreads();
-        const int cnt = min(kBS, n - t0);
+        const int cnt = min(kBS, j1 - t0);
         for (int jj = 0; jj < cnt; ++jj) {
             const double4 q = sx[jj];
             const double4 l = sl[jj];
@@ -77,12 +103,7 @@ __global__ __launch_bounds__(kBS) void k_vjp64(const double *__restrict__ pos,
         }
         __syncthreads();
     }
-    if (ok) {
-        gpos[3 * i] = G * gx;
-        gpos[3 * i + 1] = G * gy;
-        gpos[3 * i + 2] = G * gz;
-        gmass[i] = -G * gm;
-    }
+    if (ok) finish_vjp(i, gx, gy, gz, gm, gpos, gmass, n, G, part);
 }
 
 __global__ __launch_bounds__(kBS) void k_vjp32c(const double *__restrict__ pos,
@@ -90,7 +111,7 @@ __global__ __launch_bounds__(kBS) void k_vjp32c(const double *__restrict__ pos,
                                                 const double *__restrict__ lam,
                                                 double *__restrict__ gpos,
                                                 double *__restrict__ gmass, int n, double G,
-                                                double eps2)
+                                                double eps2, int jlen, double *__restrict__ part)
 {
     __shared__ float4 sh[kBS];
     __shared__ float4 slo[kBS];
@@ -108,9 +129,11 @@ __global__ __launch_bounds__(kBS) void k_vjp32c(const double *__restrict__ pos,
     const float lzi = ok ? static_cast<float>(lam[3 * i + 2]) : 0.f;
     const float eps2f = static_cast<float>(eps2);
     float s[4] = {0.f, 0.f, 0.f, 0.f}, c[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int t0 = 0; t0 < n; t0 += kBS) {
+    const int j0 = blockIdx.y * jlen;
+    const int j1 = min(n, j0 + jlen);
+    for (int t0 = j0; t0 < j1; t0 += kBS) {
         const int j = t0 + tid;
-        if (j < n) {
+        if (j < j1) {
             float hx, lx, hy, ly, hz, lz;
             split_df(pos[3 * j], hx, lx);
             split_df(pos[3 * j + 1], hy, ly);
@@ -121,7 +144,7 @@ __global__ __launch_bounds__(kBS) void k_vjp32c(const double *__restrict__ pos,
                                   static_cast<float>(lam[3 * j + 2]), 0.f);
         }
         __syncthreads();
-        const int cnt = min(kBS, n - t0);
+        const int cnt = min(kBS, j1 - t0);
         for (int c0 = 0; c0 < cnt; c0 += kChunk) {
             const int c1 = min(c0 + kChunk, cnt);
             float p[4] = {0.f, 0.f, 0.f, 0.f};
@@ -157,11 +180,33 @@ __global__ __launch_bounds__(kBS) void k_vjp32c(const double *__restrict__ pos,
         }
         __syncthreads();
     }
-    if (ok) {
-        gpos[3 * i] = G * (static_cast<double>(s[0]) + static_cast<double>(c[0]));
-        gpos[3 * i + 1] = G * (static_cast<double>(s[1]) + static_cast<double>(c[1]));
-        gpos[3 * i + 2] = G * (static_cast<double>(s[2]) + static_cast<double>(c[2]));
-        gmass[i] = -G * (static_cast<double>(s[3]) + static_cast<double>(c[3]));
+    if (ok)
+        finish_vjp(i, static_cast<double>(s[0]) + static_cast<double>(c[0]),
+                   static_cast<double>(s[1]) + static_cast<double>(c[1]),
+                   static_cast<double>(s[2]) + static_cast<double>(c[2]),
+                   static_cast<double>(s[3]) + static_cast<double>(c[3]), gpos, gmass, n, G,
+                   part);
+}
+
+// Adds the ns slice sums of each particle in slice order (FP64) and scales
+// them as the single-slice kernels do.
+__global__ __launch_bounds__(256) void k_vjp_reduce(const double *__restrict__ part, int ns,
+                                                    int n, double *__restrict__ gpos,
+                                                    double *__restrict__ gmass, double G)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = part[static_cast<size_t>(c) * n + i];
+        for (int s = 1; s < ns; ++s) {
+            const double *p = part + static_cast<size_t>(s) * 4 * n;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = v[c] + p[static_cast<size_t>(c) * n + i];
+        }
+        gpos[3 * i] = G * v[0];
+        gpos[3 * i + 1] = G * v[1];
+        gpos[3 * i + 2] = G * v[2];
+        gmass[i] = -G * v[3];
     }
 }
 
@@ -182,18 +227,33 @@ extern "C" int rg_direct_vjp(const double *pos, const double *mass, const double
         RG_REQUIRE(!rg_overlap(g_pos, b3, ins[k], insz[k]) && !rg_overlap(g_mass, b1, ins[k], insz[k]),
                    RG_ERR_ALIAS, "outputs alias inputs");
     RG_REQUIRE(!rg_overlap(g_pos, b3, g_mass, b1), RG_ERR_ALIAS, "g_pos aliases g_mass");
+    RG_REQUIRE(precision == RG_FP64 || precision == RG_FP32C || precision == RG_FP32, RG_ERR_ARG,
+               "unknown precision %d", precision);
     const int ni = static_cast<int>(n);
-    const int grid = (ni + kBS - 1) / kBS;
-    hipStream_t st = rg::as_stream(stream);
-    if (precision == RG_FP64) {
-        k_vjp64<<<grid, kBS, 0, st>>>(pos, mass, lam, g_pos, g_mass, ni, G, eps2);
-    } else if (precision == RG_FP32C || precision == RG_FP32) {
-        k_vjp32c<<<grid, kBS, 0, st>>>(pos, mass, lam, g_pos, g_mass, ni, G, eps2);
-    } else {
-        rg::set_error("unknown precision %d", precision);
-        return RG_ERR_ARG;
+    const int ib = (ni + kBS - 1) / kBS;
+    // The VJP has no reference order to keep: every precision may split j.
+    int jlen = ni;
+    const int ns = rg::direct_slices(ni, &jlen);
+    double *part = nullptr;
+    if (ns > 1) {
+        void *scratch = nullptr;
+        const int rc = rg::device_scratch(sizeof(double) * 4 * static_cast<size_t>(ns) * ni,
+                                          &scratch);
+        if (rc != RG_OK) return rc;
+        part = static_cast<double *>(scratch);
     }
+    const dim3 grid(ib, ns);
+    hipStream_t st = rg::as_stream(stream);
+    if (precision == RG_FP64)
+        k_vjp64<<<grid, kBS, 0, st>>>(pos, mass, lam, g_pos, g_mass, ni, G, eps2, jlen, part);
+    else
+        k_vjp32c<<<grid, kBS, 0, st>>>(pos, mass, lam, g_pos, g_mass, ni, G, eps2, jlen, part);
     RG_CHECK_LAUNCH();
+    if (part) {
+        const int rg_grid = std::min((ni + 255) / 256, 4096);
+        k_vjp_reduce<<<rg_grid, 256, 0, st>>>(part, ns, ni, g_pos, g_mass, G);
+        RG_CHECK_LAUNCH();
+    }
     return RG_OK;
 }
 

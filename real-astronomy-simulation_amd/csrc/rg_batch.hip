@@ -16,7 +16,8 @@
 // The pair arithmetic and the per-particle summation order are the direct
 // kernels' (rg_direct.hip): for each i the j-loop runs over the system in
 // order with the same 32-term compensation chunks, so a batched force is
-// bitwise the per-system rg_direct_acc result for the same precision.
+// bitwise the per-system rg_direct_acc result for the same precision when
+// that call runs unsplit (rg_set_param("direct_split", 1); FP64 always is).
 // Steps can be captured into a hipGraph (no memset nodes: the counter is
 // reset by a kernel).
 #include <algorithm>

@@ -15,6 +15,7 @@
 // fused is written as fma()/fmaf().  The FP64 kernels keep the reference's
 // expression order with IEEE 1.0/sqrt, so each particle's FP64 result is the
 // same double the sequential-j CPU evaluation gives.
+#include <algorithm>
 #include <cfloat>
 
 #include "rg_internal.h"
@@ -23,8 +24,18 @@ namespace {
 
 constexpr int kBS = 256;      // threads per block = j-tile length
 constexpr int kChunk = 32;    // FP32C: pair terms per partial before the two-sum fold
+// j-split of the FP32/FP32C sums.  Slices per i-block by default, and the
+// most slice-particles (slices x n) the FP64 slice sums may hold (192 MB).
+// Measured on one MI355X, FP32C, against the unsplit sum: n = 16,384
+// 0.120 ms against 0.894; 65,536 1.69 against 3.59; 262,144 26.1 against
+// 30.7 (more resident waves per SIMD hide the LDS and rsqrt latency); 32
+// slices were fastest or within 0.3% of it from n = 16,384 to 262,144.
+constexpr int kDefaultSplit = 32;
+constexpr int64_t kMaxSliceParticles = int64_t(1) << 23;
+constexpr int kMaxSplit = 256;
 
 int g_direct_ipt = 0;         // 0 = heuristic; else i-particles per thread (1, 2, 4)
+int g_direct_split = 0;       // 0 = by n; else j-slices per i-block (FP32/FP32C)
 int g_fp32c_d64 = 0;          // FP32C pair difference: 0 hi/lo FP32, 1 FP64 then rounded
 
 __device__ __forceinline__ void split_df(double x, float &hi, float &lo)
@@ -104,14 +115,52 @@ __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
     e = (a - (s - bb)) + (b - bb);
 }
 
+int elementwise_grid(int m);
+
+// What happens to particle i's finished sum: stored, added to acc (with the
+// analytic terms first for kAddExt), then the closing kick when vel is set.
+__device__ __forceinline__ void finish_particle(int i, double ax, double ay, double az,
+                                                const double *__restrict__ pos,
+                                                double *__restrict__ acc,
+                                                double *__restrict__ vel, double G,
+                                                double dt_half, const Epilogue &epi)
+{
+    if (epi.mode != kStore) {  // uniform across the grid
+        if (epi.mode == kAddExt) {
+            double av[3] = {ax, ay, az};
+            ext_terms(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], G, epi, av, false);
+            ax = av[0]; ay = av[1]; az = av[2];
+        }
+        ax = acc[3 * i] + ax;
+        ay = acc[3 * i + 1] + ay;
+        az = acc[3 * i + 2] + az;
+    }
+    acc[3 * i] = ax;
+    acc[3 * i + 1] = ay;
+    acc[3 * i + 2] = az;
+    if (vel) {  // leapfrog.py:14, v += a * dt_half
+        vel[3 * i] = vel[3 * i] + ax * dt_half;
+        vel[3 * i + 1] = vel[3 * i + 1] + ay * dt_half;
+        vel[3 * i + 2] = vel[3 * i + 2] + az * dt_half;
+    }
+}
+
 // MODE 0: FP64, reference order.  MODE 1: FP32.  MODE 2: FP32 on hi/lo
-// positions with chunked two-sum compensation.
+// positions with chunked two-sum compensation.  MODE 3: FP32C with the pair
+// difference formed in FP64.
+//
+// blockIdx.y is the j-slice [y*jlen, min(n, (y+1)*jlen)).  With one slice
+// (part == nullptr) the block finishes its particles itself; otherwise it
+// writes its slice's sum as FP64 to part[(y*3 + axis)*n + i] and
+// k_direct_reduce adds the slices in order.  FP64 always runs one slice, so
+// its sum keeps the sequential j order of gravity.py:21.
 template <int MODE, int IPT>
 __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
                                                 const double *__restrict__ mass,
                                                 double *__restrict__ acc,
                                                 double *__restrict__ vel, int n, double G,
-                                                double eps2, double dt_half, Epilogue epi)
+                                                double eps2, double dt_half, Epilogue epi,
+                                                int jlen, double *__restrict__ part)
 {
     using T4 = typename std::conditional<MODE == 0 || MODE == 3, double4, float4>::type;
     __shared__ T4 s_hi[kBS];
@@ -143,10 +192,12 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
         cx[k] = cy[k] = cz[k] = 0.f;
     }
     const float eps2f = static_cast<float>(eps2);
+    const int j0 = blockIdx.y * jlen;
+    const int j1 = min(n, j0 + jlen);
 
-    for (int t0 = 0; t0 < n; t0 += kBS) {
+    for (int t0 = j0; t0 < j1; t0 += kBS) {
         const int j = t0 + tid;
-        if (j < n) {
+        if (j < j1) {
             const double qx = pos[3 * j], qy = pos[3 * j + 1], qz = pos[3 * j + 2];
             const double qm = mass[j];
             if constexpr (MODE == 0) {
@@ -166,7 +217,7 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
             }
         }
         __syncthreads();
-        const int cnt = min(kBS, n - t0);
+        const int cnt = min(kBS, j1 - t0);
         if constexpr (MODE == 0) {
             for (int jj = 0; jj < cnt; ++jj) {
                 const double4 q = s_hi[jj];
@@ -288,24 +339,34 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
             ay = static_cast<double>(sy[k]) + static_cast<double>(cy[k]);
             az = static_cast<double>(sz[k]) + static_cast<double>(cz[k]);
         }
-        if (epi.mode != kStore) {  // uniform across the grid
-            if (epi.mode == kAddExt) {
-                double av[3] = {ax, ay, az};
-                ext_terms(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], G, epi, av, false);
-                ax = av[0]; ay = av[1]; az = av[2];
-            }
-            ax = acc[3 * i] + ax;
-            ay = acc[3 * i + 1] + ay;
-            az = acc[3 * i + 2] + az;
+        if (part) {  // uniform across the grid
+            double *p = part + static_cast<size_t>(blockIdx.y) * 3 * n;
+            p[i] = ax;
+            p[n + i] = ay;
+            p[2 * n + i] = az;
+        } else {
+            finish_particle(i, ax, ay, az, pos, acc, vel, G, dt_half, epi);
         }
-        acc[3 * i] = ax;
-        acc[3 * i + 1] = ay;
-        acc[3 * i + 2] = az;
-        if (vel) {  // leapfrog.py:14, v += a * dt_half
-            vel[3 * i] = vel[3 * i] + ax * dt_half;
-            vel[3 * i + 1] = vel[3 * i + 1] + ay * dt_half;
-            vel[3 * i + 2] = vel[3 * i + 2] + az * dt_half;
+    }
+}
+
+// Adds the ns slice sums of each particle in slice order (FP64), then
+// finishes it as the single-slice kernel does.
+__global__ __launch_bounds__(256) void k_direct_reduce(const double *__restrict__ part, int ns,
+                                                       int n, const double *__restrict__ pos,
+                                                       double *__restrict__ acc,
+                                                       double *__restrict__ vel, double G,
+                                                       double dt_half, Epilogue epi)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double ax = part[i], ay = part[n + i], az = part[2 * n + i];
+        for (int s = 1; s < ns; ++s) {
+            const double *p = part + static_cast<size_t>(s) * 3 * n;
+            ax = ax + p[i];
+            ay = ay + p[n + i];
+            az = az + p[2 * n + i];
         }
+        finish_particle(i, ax, ay, az, pos, acc, vel, G, dt_half, epi);
     }
 }
 
@@ -317,25 +378,41 @@ int launch_direct(const double *pos, const double *mass, double *acc, double *ve
     // kernels are VALU-bound; more i per lane only adds register pressure).
     const int ipt = g_direct_ipt == 0 ? 1 : g_direct_ipt;
     const int per_block = kBS * ipt;
-    const int grid = (n + per_block - 1) / per_block;
+    const int ib = (n + per_block - 1) / per_block;
+    int jlen = n;
+    const int ns = MODE == 0 ? 1 : rg::direct_slices(n, &jlen);
+    double *part = nullptr;
+    if (ns > 1) {
+        void *scratch = nullptr;
+        const int rc = rg::device_scratch(sizeof(double) * 3 * static_cast<size_t>(ns) * n,
+                                          &scratch);
+        if (rc != RG_OK) return rc;
+        part = static_cast<double *>(scratch);
+    }
+    const dim3 grid(ib, ns);
     switch (ipt) {
     case 1:
         k_direct<MODE, 1><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
-                                                  epi);
+                                                  epi, jlen, part);
         break;
     case 2:
         k_direct<MODE, 2><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
-                                                  epi);
+                                                  epi, jlen, part);
         break;
     case 4:
         k_direct<MODE, 4><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
-                                                  epi);
+                                                  epi, jlen, part);
         break;
     default:
         rg::set_error("direct_ipt must be 1, 2 or 4 (got %d)", ipt);
         return RG_ERR_ARG;
     }
     RG_CHECK_LAUNCH();
+    if (part) {
+        k_direct_reduce<<<elementwise_grid(n), 256, 0, st>>>(part, ns, n, pos, acc, vel, G,
+                                                             dt_half, epi);
+        RG_CHECK_LAUNCH();
+    }
     return RG_OK;
 }
 
@@ -502,6 +579,25 @@ __global__ __launch_bounds__(kEB) void k_energy_final(const double *__restrict__
 }
 
 }  // namespace
+
+int rg::direct_slices(int n, int *jlen)
+{
+    int ns = g_direct_split;
+    if (ns == 0)
+        ns = static_cast<int>(std::min<int64_t>(kDefaultSplit, kMaxSliceParticles / n));
+    ns = std::max(1, std::min(ns, (n + kBS - 1) / kBS));  // each slice holds a tile
+    const int per = (n + ns - 1) / ns;
+    const int len = (per + kBS - 1) / kBS * kBS;
+    *jlen = len;
+    return (n + len - 1) / len;
+}
+
+int rg::rg_direct_set_split_impl(int64_t v)
+{
+    RG_REQUIRE(v >= 0 && v <= kMaxSplit, RG_ERR_ARG, "direct_split must be in [0, %d]", kMaxSplit);
+    g_direct_split = static_cast<int>(v);
+    return RG_OK;
+}
 
 int rg::rg_direct_set_ipt_impl(int64_t v)
 {

@@ -19,8 +19,14 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 // Returns RG_OK and sets *ptr, or an error status.
 int device_scratch(size_t bytes, void **ptr);
 
+// j-slices of an FP32/FP32C pair sum over n particles: sets *jlen (a
+// multiple of the 256-particle tile) and returns the slice count,
+// ceil(n / *jlen).  Used by the direct sum and its adjoint.
+int direct_slices(int n, int *jlen);
+
 // Tunables behind rg_set_param.
 int rg_direct_set_ipt_impl(int64_t v);
+int rg_direct_set_split_impl(int64_t v);
 int rg_direct_set_fp32c_d64_impl(int64_t v);
 int rg_bh_set_walk_block_impl(int64_t v);
 int rg_bh_set_walk_mode_impl(int64_t v);

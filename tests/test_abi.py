@@ -45,6 +45,9 @@ def test_version_and_error_without_gpu():
     assert b"null" in L.rg_last_error()
     assert L.rg_set_param(b"direct_ipt", 3) == _lib.RG_ERR_ARG
     assert L.rg_set_param(b"nope", 1) == _lib.RG_ERR_ARG
+    assert L.rg_set_param(b"direct_split", 257) == _lib.RG_ERR_ARG
+    assert L.rg_set_param(b"direct_split", -1) == _lib.RG_ERR_ARG
+    assert L.rg_set_param(b"direct_split", 0) == _lib.RG_OK
     h = ctypes.c_void_p()
     assert L.rg_bh_create(1, 0.5, 0.1, 1.0, 21, 0, ctypes.byref(h)) == _lib.RG_ERR_CAPACITY
     assert L.rg_bh_create(100, -1.0, 0.1, 1.0, 21, 0, ctypes.byref(h)) == _lib.RG_ERR_ARG

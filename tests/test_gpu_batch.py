@@ -1,5 +1,6 @@
 """Batching form on the GPU: a batched force is bitwise the per-system
-direct kernel (and, in FP64, the oracle's gravity.py:13-26 restatement);
+direct kernel run unsplit (and, in FP64, the oracle's gravity.py:13-26
+restatement);
 batched KDK steps, hipGraph replays and every launch shape agree bitwise;
 per-system FP64 energies match the oracle."""
 import numpy as np
@@ -7,6 +8,16 @@ import pytest
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def unsplit():
+    """The per-system direct sums with one j-slice: the batched kernel keeps
+    that summation order."""
+    from rgsim import _lib
+    _lib.set_param("direct_split", 1)
+    yield
+    _lib.set_param("direct_split", 0)
 
 
 def small_systems(sizes=(1, 2, 37, 256, 300, 1000, 129), G=1.0):
@@ -22,7 +33,7 @@ def small_systems(sizes=(1, 2, 37, 256, 300, 1000, 129), G=1.0):
 
 @pytest.mark.parametrize("precision", ["fp32c", "fp64"])
 @pytest.mark.parametrize("tile", [64, 128, 256])
-def test_batch_acc_equals_per_system_direct(orc, precision, tile):
+def test_batch_acc_equals_per_system_direct(orc, unsplit, precision, tile):
     from rgsim.batch import Ensemble
     from rgsim.solvers import gravity
     print(f"batch acc {precision} tile={tile}", flush=True)
@@ -44,7 +55,7 @@ def test_batch_acc_equals_per_system_direct(orc, precision, tile):
 
 
 @pytest.mark.parametrize("precision", ["fp32c", "fp64"])
-def test_batch_steps_equal_per_system_steps(precision):
+def test_batch_steps_equal_per_system_steps(unsplit, precision):
     from rgsim.batch import Ensemble
     systems = small_systems()
     e1 = Ensemble(systems, precision=precision)

@@ -46,6 +46,38 @@ def test_engine_kdk_bitexact_vs_oracle(orc):
     assert eng.step_count == 10 and eng.time == pytest.approx(1e-3)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 0.0), ("fp32c", 1e-7), ("fp32", 1e-3)])
+def test_plummer4096_kdk_100_steps(orc, precision, tol):
+    """The small plain-method workload: Plummer N=4096, M=1, G=1, a=1,
+    seed 42, eps 0.01, KDK dt 1e-3 for 100 steps (engine.py:108-116 over
+    gravity.py:13-26).  FP64 is bit-exact against the oracle's trajectory;
+    FP32C and FP32 stay within tol of it (relative to the position scale);
+    the FP64 energy drift stays under verify_phase1.py:50's 1e-3."""
+    from rgsim.engine import Engine
+    from rgsim.ic import make_plummer
+    print(f"plummer4096 kdk {precision}", flush=True)
+    s = make_plummer(n=4096, total_mass=1.0, scale_radius=1.0, seed=42, G=1.0)
+    eng = Engine(softening=0.01, precision=precision, G=1.0)
+    eng.load_state(s)
+    _, _, e0 = eng.energies()
+    for _ in range(100):
+        eng.step(1e-3)
+    _, _, e1 = eng.energies()
+    out = eng.to_state()
+    pos, vel, mass = s.pos, s.vel, s.mass
+    acc = orc.direct_acc(pos, mass, 1.0, 1e-4)
+    for _ in range(100):
+        pos, vel, acc = orc.kdk_step(pos, vel, acc, mass, 1e-3, 1.0, 1e-4)
+    drift = abs((e1 - e0) / e0)
+    print(f"plummer4096 {precision}: |dE/E| {drift:.2e}, "
+          f"max |dx| {np.abs(out.pos - pos).max():.2e}")
+    if precision == "fp64":
+        assert np.array_equal(out.pos, pos) and np.array_equal(out.vel, vel)
+    else:
+        assert np.abs(out.pos - pos).max() <= tol * np.abs(pos).max()
+    assert drift < 1e-3
+
+
 def test_verify_bh_galaxy_restated(orc):
     """verify_bh_galaxy.py:17-55 with the disk+bulge generator in place of
     the SPH galaxy scenario: mean relative force error < 5% at theta 0.6,
