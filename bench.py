@@ -221,10 +221,12 @@ def bench_direct(n=262144, evals=3, kdk_steps=50):
         log(f"direct {prec}: {ms:.2f} ms/eval, {gint:.1f} Gint/s")
     out["speedup_fp32c_vs_fp64"] = out["fp64"]["ms_per_eval"] / out["fp32c"]["ms_per_eval"]
     out["speedup_fp32_vs_fp64"] = out["fp64"]["ms_per_eval"] / out["fp32"]["ms_per_eval"]
-    # the FP32/FP32C sums above run split over j (32 slices at this n, added
-    # in order in FP64); the same FP32C sum with one slice, for comparison
+    # the FP32/FP32C sums above run split over j (64 slices at this n, added
+    # in order in FP64, four i-particles per lane for FP32C); the same FP32C
+    # sum with one slice and one i-particle per lane, for comparison
     from rgsim import _lib
     _lib.set_param("direct_split", 1)
+    _lib.set_param("direct_ipt", 1)
     try:
         acc = torch.empty_like(pos)
         gravity.compute_acc_direct(pos, mass, acc, n, 1.0, eps2, "fp32c")
@@ -237,8 +239,9 @@ def bench_direct(n=262144, evals=3, kdk_steps=50):
         torch.cuda.synchronize()
     finally:
         _lib.set_param("direct_split", 0)
+        _lib.set_param("direct_ipt", 0)
     ms1 = e0.elapsed_time(e1) / evals
-    out["fp32c_unsplit"] = {"ms_per_eval": ms1, "Gint_per_s": n * n / (ms1 * 1e-3) / 1e9,
+    out["fp32c_one_slice_one_per_lane"] = {"ms_per_eval": ms1, "Gint_per_s": n * n / (ms1 * 1e-3) / 1e9,
                             "force_err_vs_fp64": rel_err(acc.cpu().numpy(), ref)}
     # precision form: 50 KDK steps dt 5e-4, FP64 energy and momentum
     eng = Engine(softening=0.005, precision="fp32c", G=1.0)

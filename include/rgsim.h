@@ -12,9 +12,10 @@
  *    stream-ordered and asynchronous unless stated otherwise.
  *  - Scratch memory: a BH handle owns its scratch.  rg_energy,
  *    rg_morton_codes, and the FP32/FP32C direct sums and rg_direct_vjp when
- *    they split j (from n = 257 up by default; at most 2^23 x 24 B) use a
- *    per-device library buffer, so two of those calls that overlap on one
- *    device (different streams, host threads) race.
+ *    they split j (from n = 257 up by default; at most 2^24 x 32 B) use a
+ *    library buffer per (device, stream), held until the process ends.
+ *    Calls on different streams never share it; calls issued to one stream
+ *    from several host threads at once race.
  *  - `precision`: RG_FP64 reproduces the reference arithmetic; RG_FP32 is
  *    plain FP32 pair arithmetic; RG_FP32C is FP32 pair arithmetic on
  *    double-float (hi+lo) positions with compensated accumulation.
@@ -53,13 +54,14 @@ int rg_version(void);
 const char *rg_last_error(void);
 
 /* Launch-shape tunables (no reference counterpart; used by the autotuner):
- *   "direct_ipt"  i-particles per lane in the direct kernels: 0 (heuristic),
- *                 1, 2 or 4
+ *   "direct_ipt"  i-particles per lane in the direct kernels: 0 (by precision
+ *                 and n: FP64 1; FP32C 4 from n = 65,536; FP32 2 from
+ *                 n = 131,072; else 1), 1, 2 or 4
  *   "direct_split" j-slices per i-block of the FP32/FP32C direct sums and of
- *                 rg_direct_vjp: 0 (default: 32, fewer when 32 x n would
- *                 pass 2^23, i.e. above n = 262,144) or 1..256, and never
- *                 more than n/256 (a slice holds at least one 256-particle
- *                 tile).  Slice sums are added in slice order in FP64, so
+ *                 rg_direct_vjp: 0 (default: 32 with one i-particle per
+ *                 lane, 64 with more, fewer when slices x n would pass 2^24)
+ *                 or 1..256, and never more than n/256 (a slice holds at
+ *                 least one 256-particle tile).  Slice sums are added in slice order in FP64, so
  *                 results are deterministic for a given n and setting; 1
  *                 gives the unsplit sum.  FP64 direct sums are never split
  *                 (they keep the reference's j order).

@@ -233,17 +233,17 @@ extern "C" int rg_direct_vjp(const double *pos, const double *mass, const double
     const int ib = (ni + kBS - 1) / kBS;
     // The VJP has no reference order to keep: every precision may split j.
     int jlen = ni;
-    const int ns = rg::direct_slices(ni, &jlen);
+    const int ns = rg::direct_slices(ni, 1, &jlen);
+    hipStream_t st = rg::as_stream(stream);
     double *part = nullptr;
     if (ns > 1) {
         void *scratch = nullptr;
         const int rc = rg::device_scratch(sizeof(double) * 4 * static_cast<size_t>(ns) * ni,
-                                          &scratch);
+                                          st, &scratch);
         if (rc != RG_OK) return rc;
         part = static_cast<double *>(scratch);
     }
     const dim3 grid(ib, ns);
-    hipStream_t st = rg::as_stream(stream);
     if (precision == RG_FP64)
         k_vjp64<<<grid, kBS, 0, st>>>(pos, mass, lam, g_pos, g_mass, ni, G, eps2, jlen, part);
     else

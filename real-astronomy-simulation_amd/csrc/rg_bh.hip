@@ -2158,7 +2158,7 @@ extern "C" int rg_morton_codes(const double *pos, int64_t n, int bits, uint64_t 
     RG_REQUIRE(bits >= 1 && bits <= 21, RG_ERR_ARG, "bits must be in [1, 21]");
     void *scratch = nullptr;
     const int ni = static_cast<int>(n);
-    const int st = rg::device_scratch(8 * bbox_part_doubles(ni), &scratch);
+    const int st = rg::device_scratch(8 * bbox_part_doubles(ni), rg::as_stream(stream), &scratch);
     if (st != RG_OK) return st;
     double *box = static_cast<double *>(scratch);
     return launch_bbox_morton(pos, ni, bits, box + 8, box, nullptr, keys, 0, nullptr, nullptr,

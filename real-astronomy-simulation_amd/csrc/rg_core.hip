@@ -2,8 +2,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
-#include <vector>
+#include <utility>
 
 #include "rg_internal.h"
 
@@ -15,7 +16,8 @@ struct Scratch {
     size_t bytes = 0;
 };
 std::mutex g_scratch_mu;
-std::vector<Scratch> g_scratch;
+// one buffer per (device, stream): calls on different streams never share one
+std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 }  // namespace
 
 namespace rg {
@@ -35,13 +37,12 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line)
     return RG_ERR_HIP;
 }
 
-int device_scratch(size_t bytes, void **ptr)
+int device_scratch(size_t bytes, hipStream_t stream, void **ptr)
 {
     int dev = 0;
     RG_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_scratch_mu);
-    if (g_scratch.size() <= static_cast<size_t>(dev)) g_scratch.resize(dev + 1);
-    Scratch &s = g_scratch[dev];
+    Scratch &s = g_scratch[std::make_pair(dev, stream)];
     if (s.bytes < bytes) {
         // Grow geometrically.  hipFree waits for the device, so a kernel
         // still reading the old buffer finishes first.

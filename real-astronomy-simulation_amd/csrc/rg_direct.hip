@@ -24,15 +24,24 @@ namespace {
 
 constexpr int kBS = 256;      // threads per block = j-tile length
 constexpr int kChunk = 32;    // FP32C: pair terms per partial before the two-sum fold
-// j-split of the FP32/FP32C sums.  Slices per i-block by default, and the
-// most slice-particles (slices x n) the FP64 slice sums may hold (192 MB).
-// Measured on one MI355X, FP32C, against the unsplit sum: n = 16,384
-// 0.120 ms against 0.894; 65,536 1.69 against 3.59; 262,144 26.1 against
-// 30.7 (more resident waves per SIMD hide the LDS and rsqrt latency); 32
-// slices were fastest or within 0.3% of it from n = 16,384 to 262,144.
-constexpr int kDefaultSplit = 32;
-constexpr int64_t kMaxSliceParticles = int64_t(1) << 23;
+// j-split of the FP32/FP32C sums.  Slices per i-block by default (32 with
+// one i-particle per lane, 64 with more), and the most slice-particles
+// (slices x n) the FP64 slice sums may hold (2^24: 402 MB of 3 doubles).
+// Measured on one MI355X, FP32C, one i per lane, against the unsplit sum:
+// n = 16,384 0.120 ms against 0.894; 65,536 1.69 against 3.59; 262,144 26.1
+// against 30.7 (more resident waves per SIMD hide the LDS and rsqrt
+// latency).
+constexpr int kSplitOne = 32;
+constexpr int kSplitMany = 64;
+constexpr int64_t kMaxSliceParticles = int64_t(1) << 24;
 constexpr int kMaxSplit = 256;
+// i-particles per lane by default.  FP64: 1 (108.3 ms with 2 at 262,144,
+// 104.8 with 1).  FP32C: 4 from n = 65,536 (each LDS broadcast feeds four
+// pairs; 262,144: 25.1 ms with 4 and 64 slices, 26.4 with 1 and 32;
+// 65,536: 1.66 against 1.70), 1 below (16,384: 0.121 against 0.145).
+// FP32: 2 from n = 131,072 (262,144: 18.63 ms against 18.91), 1 below.
+constexpr int kIpt4From = 65536;
+constexpr int kIpt2From = 131072;
 
 int g_direct_ipt = 0;         // 0 = heuristic; else i-particles per thread (1, 2, 4)
 int g_direct_split = 0;       // 0 = by n; else j-slices per i-block (FP32/FP32C)
@@ -374,18 +383,21 @@ template <int MODE>
 int launch_direct(const double *pos, const double *mass, double *acc, double *vel, int n,
                   double G, double eps2, double dt_half, const Epilogue &epi, hipStream_t st)
 {
-    // One i-particle per lane measured fastest in every precision (the
-    // kernels are VALU-bound; more i per lane only adds register pressure).
-    const int ipt = g_direct_ipt == 0 ? 1 : g_direct_ipt;
+    int ipt = g_direct_ipt;
+    if (ipt == 0) {
+        ipt = 1;
+        if ((MODE == 2 || MODE == 3) && n >= kIpt4From) ipt = 4;
+        if (MODE == 1 && n >= kIpt2From) ipt = 2;
+    }
     const int per_block = kBS * ipt;
     const int ib = (n + per_block - 1) / per_block;
     int jlen = n;
-    const int ns = MODE == 0 ? 1 : rg::direct_slices(n, &jlen);
+    const int ns = MODE == 0 ? 1 : rg::direct_slices(n, ipt, &jlen);
     double *part = nullptr;
     if (ns > 1) {
         void *scratch = nullptr;
         const int rc = rg::device_scratch(sizeof(double) * 3 * static_cast<size_t>(ns) * n,
-                                          &scratch);
+                                          st, &scratch);
         if (rc != RG_OK) return rc;
         part = static_cast<double *>(scratch);
     }
@@ -580,11 +592,12 @@ __global__ __launch_bounds__(kEB) void k_energy_final(const double *__restrict__
 
 }  // namespace
 
-int rg::direct_slices(int n, int *jlen)
+int rg::direct_slices(int n, int ipt, int *jlen)
 {
     int ns = g_direct_split;
     if (ns == 0)
-        ns = static_cast<int>(std::min<int64_t>(kDefaultSplit, kMaxSliceParticles / n));
+        ns = static_cast<int>(
+            std::min<int64_t>(ipt == 1 ? kSplitOne : kSplitMany, kMaxSliceParticles / n));
     ns = std::max(1, std::min(ns, (n + kBS - 1) / kBS));  // each slice holds a tile
     const int per = (n + ns - 1) / ns;
     const int len = (per + kBS - 1) / kBS * kBS;
@@ -715,10 +728,10 @@ extern "C" int rg_energy(const double *pos, const double *vel, const double *mas
     const int ni = static_cast<int>(n);
     const int nb = (ni + kEB - 1) / kEB;
     void *scratch = nullptr;
-    const int st = rg::device_scratch(sizeof(double) * 5 * static_cast<size_t>(nb), &scratch);
+    hipStream_t s = rg::as_stream(stream);
+    const int st = rg::device_scratch(sizeof(double) * 5 * static_cast<size_t>(nb), s, &scratch);
     if (st != RG_OK) return st;
     double *part = static_cast<double *>(scratch);
-    hipStream_t s = rg::as_stream(stream);
     k_energy_partial<<<nb, kEB, 0, s>>>(pos, vel, mass, ni, G, eps2, part);
     RG_CHECK_LAUNCH();
     k_energy_final<<<1, kEB, 0, s>>>(part, nb, out5);

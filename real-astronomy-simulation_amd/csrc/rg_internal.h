@@ -15,14 +15,15 @@ void set_error(const char *fmt, ...);
 // Records a HIP failure and returns RG_ERR_HIP.
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
-// Per-device, grow-only library scratch (see the header's scratch note).
-// Returns RG_OK and sets *ptr, or an error status.
-int device_scratch(size_t bytes, void **ptr);
+// Grow-only library scratch, one buffer per (current device, stream) (see
+// the header's scratch note).  Returns RG_OK and sets *ptr, or an error
+// status.
+int device_scratch(size_t bytes, hipStream_t stream, void **ptr);
 
-// j-slices of an FP32/FP32C pair sum over n particles: sets *jlen (a
-// multiple of the 256-particle tile) and returns the slice count,
-// ceil(n / *jlen).  Used by the direct sum and its adjoint.
-int direct_slices(int n, int *jlen);
+// j-slices of an FP32/FP32C pair sum over n particles with ipt i-particles
+// per lane: sets *jlen (a multiple of the 256-particle tile) and returns the
+// slice count, ceil(n / *jlen).  Used by the direct sum and its adjoint.
+int direct_slices(int n, int ipt, int *jlen);
 
 // Tunables behind rg_set_param.
 int rg_direct_set_ipt_impl(int64_t v);

@@ -5,7 +5,9 @@ block size, SURVEYed as having no block_dim overrides).
 
 * ``tune_batch(ens)``  tile (i-particles per work item = threads per
   workgroup) x persistent grid size of the batched force kernel.
-* ``tune_direct(pos, mass, ...)``  i-particles per lane of the direct kernels.
+* ``tune_direct(pos, mass, ...)``  i-particles per lane x j-slices of the
+  direct kernels (FP64 sums are never split, so only i-particles per lane
+  are tried for them).
 * ``tune_walk(bh, pos, mass, acc)``  threads per block of the BH walk.
 
 Results can be cached in a JSON file keyed by device and workload.
@@ -72,17 +74,24 @@ def tune_batch(ens, tiles=(64, 128, 256), grids_per_cu=(1, 2, 4, 8), reps=2, cac
     return best
 
 
-def tune_direct(pos, mass, G, eps2, precision, ipts=(1, 2, 4), reps=2):
+def tune_direct(pos, mass, G, eps2, precision, ipts=(1, 2, 4), splits=(0, 8, 16, 32, 64),
+                reps=2):
     from .solvers import gravity
     n = pos.shape[0]
     acc = torch.empty_like(pos)
+    if precision == "fp64":
+        splits = (0,)
     trials = []
     for ipt in ipts:
         _lib.set_param("direct_ipt", ipt)
-        ms = _time(lambda: gravity.compute_acc_direct(pos, mass, acc, n, G, eps2, precision), reps)
-        trials.append({"ipt": ipt, "ms": ms})
+        for split in splits:
+            _lib.set_param("direct_split", split)
+            ms = _time(lambda: gravity.compute_acc_direct(pos, mass, acc, n, G, eps2, precision),
+                       reps)
+            trials.append({"ipt": ipt, "split": split, "ms": ms})
     best = min(trials, key=lambda t: t["ms"])
     _lib.set_param("direct_ipt", best["ipt"])
+    _lib.set_param("direct_split", best["split"])
     return dict(best, trials=trials)
 
 

@@ -127,3 +127,74 @@ def test_split_vjp_vs_closed_form(split, ns, precision, rtol):
     gx2, gm2 = direct_vjp(*args)
     assert torch.equal(gx, gx2) and torch.equal(gm, gm2)
 
+
+
+def test_split_on_two_streams(split):
+    """Split sums issued on two streams at once each use their own scratch:
+    both equal the same sums issued one after another on one stream."""
+    from rgsim.solvers import gravity
+    print("split two streams", flush=True)
+    systems = [plummer(n, seed=20 + n) for n in (6000, 7001)]
+    split(0)
+    bufs = []
+    for s in systems:
+        p, m = dev(s.pos), dev(s.mass)
+        bufs.append((p, m, torch.empty_like(p), torch.empty_like(p)))
+    for p, m, ref, _ in bufs:
+        gravity.compute_acc_direct(p, m, ref, len(m), 1.0, 1e-4, "fp32c")
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for _ in range(3):
+        for (p, m, _, out), st in zip(bufs, streams):
+            with torch.cuda.stream(st):
+                gravity.compute_acc_direct(p, m, out, len(m), 1.0, 1e-4, "fp32c")
+        torch.cuda.synchronize()
+        for p, m, ref, out in bufs:
+            assert torch.equal(out, ref)
+
+
+def test_direct_autotuner_keeps_results_in_tolerance(orc):
+    """tune_direct times i-particles per lane x j-slices and leaves the
+    fastest set; the sum at that setting stays within the FP32C tolerance of
+    the oracle.  FP64 is tuned over i-particles per lane only and stays
+    bit-exact."""
+    from rgsim import _lib
+    from rgsim.autotune import tune_direct
+    from rgsim.solvers import gravity
+    n = 3000
+    print(f"tune_direct n={n}", flush=True)
+    s = plummer(n, seed=31)
+    p, m = dev(s.pos), dev(s.mass)
+    ref = orc.direct_acc(s.pos, s.mass, 1.0, 1e-4)
+    try:
+        best = tune_direct(p, m, 1.0, 1e-4, "fp32c", ipts=(1, 2), splits=(0, 1, 4), reps=1)
+        assert len(best["trials"]) == 6 and best["split"] in (0, 1, 4)
+        acc = torch.empty_like(p)
+        gravity.compute_acc_direct(p, m, acc, n, 1.0, 1e-4, "fp32c")
+        assert rel_rows(acc.cpu().numpy(), ref).max() < 3e-6
+        best = tune_direct(p, m, 1.0, 1e-4, "fp64", ipts=(1, 2), reps=1)
+        assert len(best["trials"]) == 2 and best["split"] == 0
+        gravity.compute_acc_direct(p, m, acc, n, 1.0, 1e-4, "fp64")
+        assert np.array_equal(acc.cpu().numpy(), ref)
+    finally:
+        _lib.set_param("direct_ipt", 0)
+        _lib.set_param("direct_split", 0)
+
+
+@pytest.mark.parametrize("n,precision,tol_max,tol_rms", [
+    (65536, "fp32c", 1e-6, 1e-7), (70001, "fp32c", 1e-6, 1e-7), (131073, "fp32", 2e-3, 1e-4)])
+def test_default_shape_large_n(orc, n, precision, tol_max, tol_rms):
+    """The default launch shape at large n (FP32C four i-particles per lane
+    and 64 slices from n = 65,536; FP32 two per lane from 131,072), ragged
+    sizes included, against the oracle: FP32C within the 1e-6 maximum
+    relative error the precision form is held to."""
+    from rgsim.solvers import gravity
+    print(f"default shape {precision} n={n}", flush=True)
+    s = plummer(n, seed=3)
+    ref = orc.direct_acc(s.pos, s.mass, 1.0, 0.005 ** 2)
+    p, m = dev(s.pos), dev(s.mass)
+    acc = torch.empty_like(p)
+    gravity.compute_acc_direct(p, m, acc, n, 1.0, 0.005 ** 2, precision)
+    rel = rel_rows(acc.cpu().numpy(), ref)
+    print(f"  max {rel.max():.2e} rms {np.sqrt(np.mean(rel ** 2)):.2e}", flush=True)
+    assert rel.max() < tol_max and np.sqrt(np.mean(rel ** 2)) < tol_rms
