@@ -57,6 +57,12 @@ const char *rg_last_error(void);
  *   "direct_ipt"  i-particles per lane in the direct kernels: 0 (by precision
  *                 and n: FP64 1; FP32C 4 from n = 65,536; FP32 2 from
  *                 n = 131,072; else 1), 1, 2 or 4
+ *   "fp64_pipe"   waves per 64 i-particles of the FP64 direct sums: 0 (by n:
+ *                 16 up to n = 16,384, 8 below 49,152, else 1), 1 (one wave
+ *                 per 64 i-particles), 4, 8 or 16.  Every setting gives the
+ *                 same doubles (the reference's j order); with more than one
+ *                 wave, the waves compute pair terms in parallel and three
+ *                 waves add them in j order
  *   "direct_split" j-slices per i-block of the FP32/FP32C direct sums and of
  *                 rg_direct_vjp: 0 (default: 32 with one i-particle per
  *                 lane, 64 with more, fewer when slices x n would pass 2^24)

@@ -69,6 +69,7 @@ extern "C" int rg_set_param(const char *name, int64_t value)
     RG_REQUIRE(name, RG_ERR_ARG, "null name");
     if (std::strcmp(name, "direct_ipt") == 0) return rg::rg_direct_set_ipt_impl(value);
     if (std::strcmp(name, "direct_split") == 0) return rg::rg_direct_set_split_impl(value);
+    if (std::strcmp(name, "fp64_pipe") == 0) return rg::rg_direct_set_fp64_pipe_impl(value);
     if (std::strcmp(name, "fp32c_d64") == 0) return rg::rg_direct_set_fp32c_d64_impl(value);
     if (std::strcmp(name, "walk_block") == 0) return rg::rg_bh_set_walk_block_impl(value);
     if (std::strcmp(name, "walk_mode") == 0) return rg::rg_bh_set_walk_mode_impl(value);

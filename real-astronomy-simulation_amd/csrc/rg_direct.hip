@@ -42,9 +42,17 @@ constexpr int kMaxSplit = 256;
 // FP32: 2 from n = 131,072 (262,144: 18.63 ms against 18.91), 1 below.
 constexpr int kIpt4From = 65536;
 constexpr int kIpt2From = 131072;
+// FP64 below kPipeBelow runs the pipelined kernel (k_direct64_pipe): one
+// wave per 64 i-particles leaves most SIMDs idle there.  Measured on one
+// MI355X (ms per evaluation, one wave / 8 / 16 waves per 64 i-particles):
+// n = 4,096 0.512 / 0.192 / 0.155; 16,384 2.05 / 0.780 / 0.624; 20,000
+// 2.50 / 1.44 / 1.50; 32,768 4.15 / 2.41 / 2.47; 65,536 8.23 / 9.02 / 9.88.
+constexpr int kPipe16Upto = 16384;
+constexpr int kPipeBelow = 49152;
 
 int g_direct_ipt = 0;         // 0 = heuristic; else i-particles per thread (1, 2, 4)
 int g_direct_split = 0;       // 0 = by n; else j-slices per i-block (FP32/FP32C)
+int g_fp64_pipe = 0;          // FP64 waves per 64 i-particles: 0 by n, 1 off, 4, 8, 16
 int g_fp32c_d64 = 0;          // FP32C pair difference: 0 hi/lo FP32, 1 FP64 then rounded
 
 __device__ __forceinline__ void split_df(double x, float &hi, float &lo)
@@ -359,6 +367,74 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
     }
 }
 
+// FP64 in the reference's j order with W waves per 64 i-particles.  The
+// sum a_i = (((0 + t_i0) + t_i1) + ...) must be formed in j order to be the
+// reference's double, so it cannot be split over j; but the pair terms
+// t_ij = (((G m_j) inv) inv) inv * d_ij (gravity.py:22-25) are independent.
+// Each round, wave w computes the terms of S consecutive j (w*S .. w*S+S-1
+// of the round's R = W*S) for the block's 64 particles (lane = i) and
+// writes them to LDS; after the round's barrier, waves 0, 1 and 2 add the
+// round's R terms of axis x, y and z, in j order, to their running sums
+// while every wave computes the next round's terms into the other LDS
+// buffer.  One barrier per round keeps the buffers apart: a buffer is
+// rewritten two rounds later, after the barrier that the adding waves reach
+// only when they are done reading it.  The j-particle data are uniform
+// across the wave (scalar loads).  Same doubles as k_direct<0>.
+template <int W, int S>
+__global__ __launch_bounds__(64 * W) void k_direct64_pipe(const double *__restrict__ pos,
+                                                         const double *__restrict__ mass,
+                                                         double *__restrict__ acc,
+                                                         double *__restrict__ vel, int n,
+                                                         double G, double eps2, double dt_half,
+                                                         Epilogue epi)
+{
+    constexpr int R = W * S;
+    __shared__ double s_term[2][R][3][64];
+    __shared__ double s_fin[3][64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 64 + lane;
+    const bool ok = i < n;
+    const double xd = ok ? pos[3 * i] : 0.0;
+    const double yd = ok ? pos[3 * i + 1] : 0.0;
+    const double zd = ok ? pos[3 * i + 2] : 0.0;
+    double sum = 0.0;  // waves 0..2: axis `wave` of particle i
+    int r = 0;
+    for (int r0 = 0; r0 < n; r0 += R, ++r) {
+        const int buf = r & 1;
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            const int jl = wave * S + q;
+            const int j = r0 + jl;
+            if (j < n) {  // uniform
+                const double qx = pos[3 * j], qy = pos[3 * j + 1], qz = pos[3 * j + 2];
+                const double qm = mass[j];
+                // gravity.py:22-25, same association, IEEE sqrt and divide.
+                const double dx = qx - xd;
+                const double dy = qy - yd;
+                const double dz = qz - zd;
+                const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                const double inv = 1.0 / sqrt(r2);
+                const double sj = G * qm * inv * inv * inv;
+                s_term[buf][jl][0][lane] = sj * dx;
+                s_term[buf][jl][1][lane] = sj * dy;
+                s_term[buf][jl][2][lane] = sj * dz;
+            }
+        }
+        __syncthreads();
+        if (wave < 3) {  // uniform per wave
+            const int cnt = min(R, n - r0);
+            const double *t = &s_term[buf][0][wave][lane];
+            for (int q = 0; q < cnt; ++q) sum = sum + t[q * 3 * 64];
+        }
+    }
+    if (wave < 3) s_fin[wave][lane] = sum;
+    __syncthreads();
+    if (wave == 0 && ok)
+        finish_particle(i, s_fin[0][lane], s_fin[1][lane], s_fin[2][lane], pos, acc, vel, G,
+                        dt_half, epi);
+}
+
 // Adds the ns slice sums of each particle in slice order (FP64), then
 // finishes it as the single-slice kernel does.
 __global__ __launch_bounds__(256) void k_direct_reduce(const double *__restrict__ part, int ns,
@@ -428,6 +504,39 @@ int launch_direct(const double *pos, const double *mass, double *acc, double *ve
     return RG_OK;
 }
 
+// Waves per 64 i-particles of the FP64 pipeline (1: the one-wave kernel).
+int fp64_pipe_waves(int n)
+{
+    if (g_fp64_pipe) return g_fp64_pipe;
+    if (g_direct_ipt) return 1;  // an explicit i-per-lane choice picks k_direct<0>
+    if (n <= kPipe16Upto) return 16;
+    return n < kPipeBelow ? 8 : 1;
+}
+
+int launch_direct64_pipe(int w, const double *pos, const double *mass, double *acc, double *vel,
+                         int n, double G, double eps2, double dt_half, const Epilogue &epi,
+                         hipStream_t st)
+{
+    const int grid = (n + 63) / 64;
+    switch (w) {
+    case 4:
+        k_direct64_pipe<4, 4><<<grid, 256, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half, epi);
+        break;
+    case 8:
+        k_direct64_pipe<8, 2><<<grid, 512, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half, epi);
+        break;
+    case 16:
+        k_direct64_pipe<16, 2><<<grid, 1024, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
+                                                      epi);
+        break;
+    default:
+        rg::set_error("fp64_pipe must be 0, 1, 4, 8 or 16 (got %d)", w);
+        return RG_ERR_ARG;
+    }
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
 int direct_common(const double *pos, const double *mass, double *acc, double *vel,
                   int64_t n, double G, double eps2, int precision, double dt_half,
                   const Epilogue &epi, void *stream)
@@ -448,7 +557,11 @@ int direct_common(const double *pos, const double *mass, double *acc, double *ve
     hipStream_t st = rg::as_stream(stream);
     const int ni = static_cast<int>(n);
     switch (precision) {
-    case RG_FP64: return launch_direct<0>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
+    case RG_FP64: {
+        const int w = fp64_pipe_waves(ni);
+        if (w > 1) return launch_direct64_pipe(w, pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
+        return launch_direct<0>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
+    }
     case RG_FP32: return launch_direct<1>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
     case RG_FP32C:
         if (g_fp32c_d64)
@@ -603,6 +716,14 @@ int rg::direct_slices(int n, int ipt, int *jlen)
     const int len = (per + kBS - 1) / kBS * kBS;
     *jlen = len;
     return (n + len - 1) / len;
+}
+
+int rg::rg_direct_set_fp64_pipe_impl(int64_t v)
+{
+    RG_REQUIRE(v == 0 || v == 1 || v == 4 || v == 8 || v == 16, RG_ERR_ARG,
+               "fp64_pipe must be 0, 1, 4, 8 or 16");
+    g_fp64_pipe = static_cast<int>(v);
+    return RG_OK;
 }
 
 int rg::rg_direct_set_split_impl(int64_t v)
