@@ -63,6 +63,10 @@ const char *rg_last_error(void);
  *                 same doubles (the reference's j order); with more than one
  *                 wave, the waves compute pair terms in parallel and three
  *                 waves add them in j order
+ *   "sort_ticket" radix-sort passes (BH keys, SPH cells): 0 (default) take
+ *                 each tile's index from its workgroup when all tiles fit on
+ *                 the GPU at once (<= 256), else from an atomic ticket; 1
+ *                 always from the ticket.  Same results either way
  *   "direct_split" j-slices per i-block of the FP32/FP32C direct sums and of
  *                 rg_direct_vjp: 0 (default: 32 with one i-particle per
  *                 lane, 64 with more, fewer when slices x n would pass 2^24)

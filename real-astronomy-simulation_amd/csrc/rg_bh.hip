@@ -544,24 +544,7 @@ __device__ void treelet_rank_tile(const int *__restrict__ tl_of, int n, int *__r
                            __HIP_MEMORY_SCOPE_AGENT);
         unsigned excl = 0;
         if (tile > 0) {
-            int j = tile - 1;
-            int spins = 0;
-            while (j >= 0) {
-                const unsigned v = __hip_atomic_load(&lb[j], __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned f = v & ~kCountMask;
-                if (f == 0u) {
-                    if (++spins > kSpinLimit) {
-                        atomicAdd(&stats[3], 1ull);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += v & kCountMask;
-                if (f == kFlagInc) break;
-                --j;
-            }
+            excl = look_back(lb, tile, 1, stats);
             __hip_atomic_store(mine, kFlagInc | (excl + total), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }

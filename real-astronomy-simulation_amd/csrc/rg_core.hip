@@ -22,6 +22,8 @@ std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 
 namespace rg {
 
+int g_sort_ticket = 0;
+
 void set_error(const char *fmt, ...)
 {
     va_list ap;
@@ -70,6 +72,11 @@ extern "C" int rg_set_param(const char *name, int64_t value)
     if (std::strcmp(name, "direct_ipt") == 0) return rg::rg_direct_set_ipt_impl(value);
     if (std::strcmp(name, "direct_split") == 0) return rg::rg_direct_set_split_impl(value);
     if (std::strcmp(name, "fp64_pipe") == 0) return rg::rg_direct_set_fp64_pipe_impl(value);
+    if (std::strcmp(name, "sort_ticket") == 0) {
+        RG_REQUIRE(value == 0 || value == 1, RG_ERR_ARG, "sort_ticket must be 0 or 1");
+        rg::g_sort_ticket = static_cast<int>(value);
+        return RG_OK;
+    }
     if (std::strcmp(name, "fp32c_d64") == 0) return rg::rg_direct_set_fp32c_d64_impl(value);
     if (std::strcmp(name, "walk_block") == 0) return rg::rg_bh_set_walk_block_impl(value);
     if (std::strcmp(name, "walk_mode") == 0) return rg::rg_bh_set_walk_mode_impl(value);

@@ -25,6 +25,10 @@ int device_scratch(size_t bytes, hipStream_t stream, void **ptr);
 // slice count, ceil(n / *jlen).  Used by the direct sum and its adjoint.
 int direct_slices(int n, int ipt, int *jlen);
 
+// Radix sort passes always take tile tickets (1) or only when their tiles
+// may not all be resident at once (0, the default).
+extern int g_sort_ticket;
+
 // Tunables behind rg_set_param.
 int rg_direct_set_ipt_impl(int64_t v);
 int rg_direct_set_split_impl(int64_t v);
