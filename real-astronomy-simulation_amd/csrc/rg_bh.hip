@@ -580,16 +580,22 @@ __global__ __launch_bounds__(256) void k_rank_preorder(
     preorder_part(up, range, n, pre, stats, blockIdx.x - ntiles, gridDim.x - ntiles);
 }
 
-// Pass 1: one workgroup per treelet (grid-stride over the list), one
-// thread per leaf.  Every node of the treelet has both children inside it:
-// node data live in LDS and the arrival counters are LDS atomics at
-// workgroup scope (acq_rel: the second arriver sees the first's data).  A
-// leaf keeps only its position and mass in LDS (its moment m * x is
-// recomputed with the same rounding, its box is the point), so the
-// workgroup needs 29 KB of LDS for the node data.  The topology of the
-// treelet's internal nodes (every internal index lies in the treelet's leaf
-// range) is staged in LDS before the climb, so a level costs LDS accesses
-// and fire-and-forget stores, not dependent global loads.
+// Pass 1: one workgroup per treelet (grid-stride over the list).  Every
+// node of the treelet has both children inside it, so the whole treelet is
+// summed in LDS.  Thread t owns leaf lo + t and internal node lo + t (the
+// treelet's internal ids all lie in its leaf range; the one id there that
+// belongs to an ancestor is skipped).  The internal nodes are summed level
+// synchronously: in each pass a node whose two children were finished in
+// earlier passes is combined, and one barrier ends the pass -- a node
+// finished in pass k carries k in s_done, and readers in pass k accept
+// only children with a smaller value, so no flag is read before the data
+// it guards is visible.  The passes number the treelet's height (at most
+// 17 at 120k).  Only the sums (mass, moment, box) are on that chain; the
+// COM divisions and the node and walk-record stores follow in parallel,
+// one node per thread.  Each node is the same expression of its children
+// (left + right) as a bottom-up climb, so the doubles do not depend on the
+// schedule.  A leaf keeps only its position and mass in LDS (its moment is
+// m * x with the same rounding, its box the point).
 template <int MODE>
 __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     const int *__restrict__ order, const double *__restrict__ pos,
@@ -606,39 +612,33 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     __shared__ double s_mom[3][C];
     __shared__ double s_mn[3][C];
     __shared__ double s_mx[3][C];
-    __shared__ int s_arrive[C];
-    __shared__ int s_L[C], s_R[C], s_par[C], s_pre[C];  // internal node lo + t
-    __shared__ int2 s_rg[C];
+    __shared__ int s_done[C];       // pass that finished internal node lo + t (0: not yet)
     const int tid = threadIdx.x;
     const int cnt = counts[3];  // treelets, in leaf order in tsorted
     for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
         const int v = tsorted[e];
         const int lo = v < n ? v : range[v - n].x;
         const int size = v < n ? 1 : range[v - n].y - lo + 1;
-        s_arrive[tid] = 0;
-        int lpar = -1, lpre = 0;
-        const double *xp = nullptr;
-        double m = 0.0;
-        if (tid < size) {
-            const int ii = lo + tid;  // internal ids run 0 .. n-2
-            if (ii < n - 1) {
-                s_L[tid] = left[ii];
-                s_R[tid] = right[ii];
-                s_rg[tid] = range[ii];
-                s_par[tid] = parent[ii + n];
-                if (MODE != 0) s_pre[tid] = wo.pre[ii + n];
-            }
-            const int p = order[ii];
-            xp = pos + 3 * p;
-            m = mass[p];
-            lpar = parent[ii];
-            if (MODE != 0) lpre = wo.pre[ii];
+        const int hi = lo + size - 1;
+        // internal node lo + tid: topology in registers
+        bool own = false;
+        int L = 0, R = 0, npre = 0;
+        int2 rg = make_int2(0, 0);
+        const int ii = lo + tid;
+        if (tid < size && ii < n - 1) {  // independent loads, issued together
+            rg = range[ii];
+            L = left[ii];
+            R = right[ii];
+            if (MODE != 0) npre = wo.pre[ii + n];
+            own = rg.x >= lo && rg.y <= hi;  // inside this treelet
         }
-        __syncthreads();
+        s_done[tid] = 0;
         if (tid < size) {
-            const int li = lo + tid;
-            const double x[3] = {xp[0], xp[1], xp[2]};
             // barnes_hut.py:129-134 (leaf init).
+            const int li = lo + tid;
+            const int p = order[li];
+            const double x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+            const double m = mass[p];
             double mom[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -647,80 +647,90 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
             }
             s_lm[tid] = m;
             write_node<MODE>(agg, wo, li, n, m, mom, x, x, x, li, li, -1,
-                             li == 0 ? -1 : left_at_split[li - 1], lpre);
-            int node = li == v ? -1 : lpar;
-            while (node >= 0) {
-                const int ii = node - n;
-                const int t = ii - lo;
-                // Only the LDS node data must be ordered with the arrival
-                // counter (the global outputs are read by later kernels), so
-                // the fences are LDS-only and do not wait for global stores.
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-                const int old = __hip_atomic_fetch_add(&s_arrive[ii - lo], 1, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-                if (old == 0) break;  // the sibling subtree is not finished; its thread continues
-                const int L = s_L[t], R = s_R[t];
-                const int2 rg = s_rg[t];
-                double mL, mR, momL[3], momR[3], mnL[3], mnR[3], mxL[3], mxR[3];
-                if (L < n) {
-                    const int sL = L - lo;
-                    mL = s_lm[sL];
+                             li == 0 ? -1 : left_at_split[li - 1], MODE != 0 ? wo.pre[li] : 0);
+        }
+        __syncthreads();
+        if (v >= n) {  // uniform: the treelet has internal nodes
+            const int troot = v - n - lo;
+            double mm = 0.0, mom[3] = {0.0, 0.0, 0.0}, mn[3] = {0.0, 0.0, 0.0},
+                   mx[3] = {0.0, 0.0, 0.0};
+            bool done = !own;
+            // at most C passes (a treelet's height is below its leaf count)
+            for (int pass = 1; pass <= C; ++pass) {
+                if (!done) {
+                    const int sL = L < n ? L - lo : L - n - lo;
+                    const int sR = R < n ? R - lo : R - n - lo;
+                    const int dL = L < n ? 0 : s_done[sL];
+                    const int dR = R < n ? 0 : s_done[sR];
+                    const bool okL = L < n || (dL > 0 && dL < pass);
+                    const bool okR = R < n || (dR > 0 && dR < pass);
+                    if (okL && okR) {
+                        double mL, mR, momL[3], momR[3], mnL[3], mnR[3], mxL[3], mxR[3];
+                        if (L < n) {
+                            mL = s_lm[sL];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        momL[a] = mL * s_lx[a][sL];
-                        mnL[a] = mxL[a] = s_lx[a][sL];
-                    }
-                } else {
-                    const int sL = L - n - lo;
-                    mL = s_m[sL];
+                            for (int a = 0; a < 3; ++a) {
+                                momL[a] = mL * s_lx[a][sL];
+                                mnL[a] = mxL[a] = s_lx[a][sL];
+                            }
+                        } else {
+                            mL = s_m[sL];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        momL[a] = s_mom[a][sL];
-                        mnL[a] = s_mn[a][sL];
-                        mxL[a] = s_mx[a][sL];
+                            for (int a = 0; a < 3; ++a) {
+                                momL[a] = s_mom[a][sL];
+                                mnL[a] = s_mn[a][sL];
+                                mxL[a] = s_mx[a][sL];
+                            }
+                        }
+                        if (R < n) {
+                            mR = s_lm[sR];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                momR[a] = mR * s_lx[a][sR];
+                                mnR[a] = mxR[a] = s_lx[a][sR];
+                            }
+                        } else {
+                            mR = s_m[sR];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                momR[a] = s_mom[a][sR];
+                                mnR[a] = s_mn[a][sR];
+                                mxR[a] = s_mx[a][sR];
+                            }
+                        }
+                        mm = mL + mR;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            mom[a] = momL[a] + momR[a];
+                            mn[a] = fmin(mnL[a], mnR[a]);
+                            mx[a] = fmax(mxL[a], mxR[a]);
+                        }
+                        s_m[tid] = mm;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            s_mom[a][tid] = mom[a];
+                            s_mn[a][tid] = mn[a];
+                            s_mx[a][tid] = mx[a];
+                        }
+                        s_done[tid] = pass;
+                        done = true;
                     }
                 }
-                if (R < n) {
-                    const int sR = R - lo;
-                    mR = s_lm[sR];
+                __syncthreads();
+                // The root finished by this pass: every thread reads the same
+                // value here, since a root finished in the next pass would
+                // carry pass + 1.
+                const int dr = s_done[troot];
+                if (dr != 0 && dr <= pass) break;
+            }
+            if (own) {
+                double c[3];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        momR[a] = mR * s_lx[a][sR];
-                        mnR[a] = mxR[a] = s_lx[a][sR];
-                    }
-                } else {
-                    const int sR = R - n - lo;
-                    mR = s_m[sR];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        momR[a] = s_mom[a][sR];
-                        mnR[a] = s_mn[a][sR];
-                        mxR[a] = s_mx[a][sR];
-                    }
-                }
-                const double mm = mL + mR;
-                double mn[3], mx[3], c[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    mom[a] = momL[a] + momR[a];
-                    mn[a] = fmin(mnL[a], mnR[a]);
-                    mx[a] = fmax(mxL[a], mxR[a]);
+                for (int a = 0; a < 3; ++a)
                     // barnes_hut.py:175-177: com = mom / mass where mass > 0.
                     c[a] = mm > 0.0 ? mom[a] / mm : 0.0;
-                }
-                const int sn = ii - lo;
-                s_m[sn] = mm;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    s_mom[a][sn] = mom[a];
-                    s_mn[a][sn] = mn[a];
-                    s_mx[a][sn] = mx[a];
-                }
-                write_node<MODE>(agg, wo, node, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
-                                 rg.x == 0 ? -1 : left_at_split[rg.x - 1],
-                                 MODE == 0 ? 0 : s_pre[t]);
-                node = node == v ? -1 : s_par[t];
+                write_node<MODE>(agg, wo, ii + n, n, mm, mom, mn, mx, c, rg.x, rg.y, R,
+                                 rg.x == 0 ? -1 : left_at_split[rg.x - 1], npre);
             }
         }
         __syncthreads();  // the LDS slots are reused by the next treelet
