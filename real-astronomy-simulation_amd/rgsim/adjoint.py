@@ -74,16 +74,23 @@ def direct_acceleration(pos, mass, G, eps, precision="fp64"):
     return DirectAcceleration.apply(pos, mass, G, float(eps) ** 2, precision)
 
 
-def _forward_segment(x, v, a, mass, steps, dt, G, eps2, precision, keep=None):
+def _forward_segment(x, v, a, mass, steps, dt, G, eps2, precision, keep=None,
+                     positions_only=False):
     """Advance (x, v, a) in place by `steps` KDK steps (the Engine's kernels);
-    if `keep` is a list, append a copy of x after each drift."""
+    if `keep` is a list, append a copy of x after each drift.  With
+    positions_only the last step stops after its drift: the positions of a
+    segment need the forces at its first `steps` positions only, so the
+    force at the segment's end (and the closing kick) is left out and v, a
+    are not advanced past the last opening kick."""
     n = x.shape[0]
     half = 0.5 * dt
-    for _ in range(steps):
+    for s in range(steps):
         leapfrog.kick_drift(x, v, a, n, half, dt)
-        gravity.compute_acc_direct_kick(x, mass, a, v, n, G, eps2, half, precision)
         if keep is not None:
             keep.append(x.clone())
+        if positions_only and s == steps - 1:
+            break
+        gravity.compute_acc_direct_kick(x, mass, a, v, n, G, eps2, half, precision)
 
 
 class KDKTrajectory(torch.autograd.Function):
@@ -126,7 +133,8 @@ class KDKTrajectory(torch.autograd.Function):
             cx, cv, ca = (t.clone() for t in ckpts[seg])
             seg_steps = min(k, steps - seg * k)
             xs = [cx.clone()]
-            _forward_segment(cx, cv, ca, m, seg_steps, dt, G, eps2, precision, keep=xs)
+            _forward_segment(cx, cv, ca, m, seg_steps, dt, G, eps2, precision, keep=xs,
+                             positions_only=True)
             for s in range(seg_steps - 1, -1, -1):
                 x_end = xs[s + 1]
                 direct_vjp(x_end, m, lam, G, eps2, precision, out=(g_pos, g_mass))
