@@ -167,7 +167,9 @@ int rg_kick_drift(double *pos, double *vel, const double *acc, int64_t n,
 /* Engine.energies (core/engine.py:119-124) -> gravity.kinetic_energy and
  * gravity.potential_energy (core/solvers/gravity.py:29-57), plus total
  * momentum.  out5 (device, 5 doubles) = {KE, PE, Px, Py, Pz}, all FP64,
- * reduced in a fixed order (bitwise reproducible).  PE excludes j == i.
+ * reduced in a fixed order (bitwise reproducible for a given n and
+ * "direct_split").  PE excludes j == i; its j range is split like the FP32
+ * direct sums.
  * pos == NULL skips the N^2 potential pass (PE = 0). */
 int rg_energy(const double *pos, const double *vel, const double *mass, int64_t n,
               double G, double eps2, double *out5, void *stream);

@@ -646,12 +646,15 @@ __device__ void block_reduce5(double v[5], double *out, int stride)
 }
 
 // Per-particle KE (gravity.py:35), PE (gravity.py:47-56, j != i) and
-// momentum; one partial row per block.
+// momentum; one partial row per block.  blockIdx.y is the j-slice
+// [y*jlen, min(n, (y+1)*jlen)) of the PE sum (the slices give the N^2 pass
+// enough blocks to fill the GPU); KE and momentum come from slice 0 only.
+// Row (y, x) of the partials is column y*gridDim.x + x.
 __global__ __launch_bounds__(kEB) void k_energy_partial(const double *__restrict__ pos,
                                                          const double *__restrict__ vel,
                                                          const double *__restrict__ mass,
                                                          int n, double G, double eps2,
-                                                         double *__restrict__ part)
+                                                         int jlen, double *__restrict__ part)
 {
     __shared__ double4 sj[kEB];
     const int tid = threadIdx.x;
@@ -661,12 +664,14 @@ __global__ __launch_bounds__(kEB) void k_energy_partial(const double *__restrict
     const double px = ok && with_pe ? pos[3 * i] : 0.0;
     const double py = ok && with_pe ? pos[3 * i + 1] : 0.0;
     const double pz = ok && with_pe ? pos[3 * i + 2] : 0.0;
+    const int j0 = blockIdx.y * jlen;
+    const int j1 = min(n, j0 + jlen);
     double acc_i = 0.0;
-    for (int t0 = 0; with_pe && t0 < n; t0 += kEB) {
+    for (int t0 = j0; with_pe && t0 < j1; t0 += kEB) {
         const int j = t0 + tid;
-        if (j < n) sj[tid] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], mass[j]);
+        if (j < j1) sj[tid] = make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], mass[j]);
         __syncthreads();
-        const int cnt = min(kEB, n - t0);
+        const int cnt = min(kEB, j1 - t0);
         for (int jj = 0; jj < cnt; ++jj) {
             const double4 q = sj[jj];
             if (t0 + jj != i) {
@@ -682,14 +687,17 @@ __global__ __launch_bounds__(kEB) void k_energy_partial(const double *__restrict
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (ok) {
         const double mi = mass[i];
-        const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
-        v[0] = 0.5 * mi * (vx * vx + vy * vy + vz * vz);
         v[1] = -0.5 * G * mi * acc_i;
-        v[2] = mi * vx;
-        v[3] = mi * vy;
-        v[4] = mi * vz;
+        if (blockIdx.y == 0) {
+            const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+            v[0] = 0.5 * mi * (vx * vx + vy * vy + vz * vz);
+            v[2] = mi * vx;
+            v[3] = mi * vy;
+            v[4] = mi * vz;
+        }
     }
-    block_reduce5(v, part + blockIdx.x, gridDim.x);
+    const int cols = gridDim.x * gridDim.y;
+    block_reduce5(v, part + blockIdx.y * gridDim.x + blockIdx.x, cols);
 }
 
 // Single block: thread t sums partials t, t+kEB, ... in order, then a tree.
@@ -848,14 +856,20 @@ extern "C" int rg_energy(const double *pos, const double *vel, const double *mas
     RG_REQUIRE(!pos || eps2 > 0.0, RG_ERR_ARG, "eps2 must be > 0 (got %g)", eps2);
     const int ni = static_cast<int>(n);
     const int nb = (ni + kEB - 1) / kEB;
+    // the PE pass is split over j like the FP32 sums (it has no j order to
+    // keep: its partials are reduced in a fixed order, to within 1e-13 of
+    // the sequential sum)
+    int jlen = ni;
+    const int ns = pos ? rg::direct_slices(ni, 1, &jlen) : 1;
     void *scratch = nullptr;
     hipStream_t s = rg::as_stream(stream);
-    const int st = rg::device_scratch(sizeof(double) * 5 * static_cast<size_t>(nb), s, &scratch);
+    const int st = rg::device_scratch(sizeof(double) * 5 * static_cast<size_t>(nb) * ns, s,
+                                      &scratch);
     if (st != RG_OK) return st;
     double *part = static_cast<double *>(scratch);
-    k_energy_partial<<<nb, kEB, 0, s>>>(pos, vel, mass, ni, G, eps2, part);
+    k_energy_partial<<<dim3(nb, ns), kEB, 0, s>>>(pos, vel, mass, ni, G, eps2, jlen, part);
     RG_CHECK_LAUNCH();
-    k_energy_final<<<1, kEB, 0, s>>>(part, nb, out5);
+    k_energy_final<<<1, kEB, 0, s>>>(part, nb * ns, out5);
     RG_CHECK_LAUNCH();
     return RG_OK;
 }
