@@ -550,6 +550,41 @@ def copy_calibration(mib=1024, reps=3):
                       "reps": reps}))
 
 
+def host_threads():
+    """Host threads for the CPU baselines: OMP_NUM_THREADS when set (the
+    GPU box sets its per-GPU CPU share there), else the CPU count, at most 16."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def cpu_baseline_direct(n=32768):
+    """The oracle's C/OpenMP FP64 direct sum (gravity.py:13-26 restated,
+    outer loop parallel) on a Plummer sample of n particles: one evaluation
+    on the host threads and one on a single thread."""
+    from oracle import np_oracle as orc
+    from rgsim.ic import make_plummer
+
+    orc.build()
+    threads = host_threads()
+    s = make_plummer(n, total_mass=1.0, scale_radius=1.0, seed=0, G=1.0)
+    out = {}
+    for label, t in (("all", threads), ("single_thread", 1)):
+        orc.set_threads(t)
+        sub = s.pos if t > 1 else s.pos[: n // 8]
+        msub = s.mass if t > 1 else s.mass[: n // 8]
+        t0 = time.perf_counter()
+        orc.direct_acc(sub, msub, 1.0, 0.005 ** 2)
+        w = time.perf_counter() - t0
+        out[label] = {"value": len(msub) ** 2 / w / 1e9, "unit": "Ginteractions/s",
+                      "sample": f"one FP64 evaluation at N={len(msub)}, {w:.2f} s on {t} thread(s)"}
+    orc.set_threads(threads)
+    return {"value": out["all"]["value"], "unit": "Ginteractions/s", "cores": threads,
+            "kind": "port", "sample": out["all"]["sample"] + " (the oracle's C/OpenMP "
+            "restatement of gravity.py:13-26)", "single_thread": out["single_thread"]}
+
+
 def cpu_baseline(steps=10):
     """The oracle's C/OpenMP restatement of the reference BH step on the host
     cores: Morton + stable sort + build + aggregation + stack walk + KDK.
@@ -560,7 +595,7 @@ def cpu_baseline(steps=10):
     from rgsim.units import G
 
     orc.build()
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     s = make_disk_bulge(n_disk=100000, n_bulge=20000, seed=1)
 
     def run(nsteps, nthreads):
@@ -634,6 +669,10 @@ def main():
     if "direct" in sections:
         log("bench: direct Plummer N=262144 ...")
         direct = bench_direct()
+        if not args.no_cpu_baseline:
+            direct["cpu_baseline_fp64"] = cpu_baseline_direct()
+            log(f"bench: direct CPU FP64 {direct['cpu_baseline_fp64']['value']:.2f} Gint/s on "
+                f"{direct['cpu_baseline_fp64']['cores']} threads")
     if "adjoint" in sections:
         log("bench: adjoint Plummer N=16384 ...")
         adjoint = bench_adjoint()
