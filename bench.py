@@ -640,7 +640,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(a run covers one GPU's share of the work)", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():

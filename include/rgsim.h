@@ -71,10 +71,10 @@ const char *rg_last_error(void);
  *                 rg_direct_vjp: 0 (default: 32 with one i-particle per
  *                 lane, 64 with more, fewer when slices x n would pass 2^24)
  *                 or 1..256, and never more than n/256 (a slice holds at
- *                 least one 256-particle tile).  Slice sums are added in slice order in FP64, so
- *                 results are deterministic for a given n and setting; 1
- *                 gives the unsplit sum.  FP64 direct sums are never split
- *                 (they keep the reference's j order).
+ *                 least one 256-particle tile).  Slice sums are added in
+ *                 slice order in FP64, so results are deterministic for a
+ *                 given n and setting; 1 gives the unsplit sum.  FP64 direct
+ *                 sums are never split (they keep the reference's j order).
  *   "walk_block"  threads per block of the BH walk: 64, 128, 256 or 512
  *   "walk_mode"   FP32 BH walk: -1 auto (default: 7), 0 one lane per
  *                 particle, 1 two lanes per particle, 2 one wave walking
@@ -334,9 +334,10 @@ typedef struct rg_batch rg_batch;
 int rg_batch_create(int64_t nsys, const int64_t *offsets, const double *G,
                     const double *eps2, const double *dt, int precision, rg_batch **out);
 
-/* Launch shape (the autotuner's knobs): tile = i-particles per work item =
- * threads per workgroup (64, 128, 256); grid = persistent workgroups
- * (0 = 4 per CU).  Drops a captured graph. */
+/* Launch shape (the autotuner's knobs): tile = i-particles per work item:
+ * 64, 128 or 256 (one per thread), 512 or 1024 (256 threads with 2 or 4
+ * each); grid = persistent workgroups (0 = 4 per CU).  Every shape gives
+ * the same doubles.  Drops a captured graph. */
 int rg_batch_set_shape(rg_batch *h, int tile, int grid);
 
 /* Accelerations of every system (overwrites acc). */

@@ -68,8 +68,12 @@ __global__ void k_batch_kick_drift(double *__restrict__ pos, double *__restrict_
 }
 
 // MODE 0: FP64 reference arithmetic; MODE 2: FP32 on hi/lo positions with
-// chunked two-sum compensation (the direct kernels' MODE numbering).
-template <int MODE, int BS>
+// chunked two-sum compensation (the direct kernels' MODE numbering).  A work
+// item is BS * IPT i-particles: thread t holds i0 + t + k * BS, k < IPT,
+// and every j-tile staged in LDS feeds all of them.  Each i-particle's sum
+// runs over its system's j in order with the 32-term chunks at the same j
+// (multiples of 32), so the result does not depend on BS or IPT.
+template <int MODE, int BS, int IPT>
 __global__ __launch_bounds__(BS) void k_batch_force(
     const double *__restrict__ pos, const double *__restrict__ mass, double *__restrict__ acc,
     double *__restrict__ vel, const int2 *__restrict__ items, int n_items,
@@ -88,17 +92,23 @@ __global__ __launch_bounds__(BS) void k_batch_force(
         if (item >= n_items) break;
         const int2 it = items[item];
         const SysInfo S = sys[it.x];
-        const int i = S.off + it.y + tid;
-        const bool ok = it.y + tid < S.n;
-        const double px = ok ? pos[3 * i] : 0.0;
-        const double py = ok ? pos[3 * i + 1] : 0.0;
-        const double pz = ok ? pos[3 * i + 2] : 0.0;
-        double axd = 0.0, ayd = 0.0, azd = 0.0;
-        float xh, xl, yh, yl, zh, zl;
-        split_df(px, xh, xl);
-        split_df(py, yh, yl);
-        split_df(pz, zh, zl);
-        float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+        double pd[IPT][3];
+        double axd[IPT], ayd[IPT], azd[IPT];
+        float xh[IPT], xl[IPT], yh[IPT], yl[IPT], zh[IPT], zl[IPT];
+        float sx[IPT], sy[IPT], sz[IPT], cx[IPT], cy[IPT], cz[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int li = it.y + tid + k * BS;
+            const bool ok = li < S.n;
+            const int i = S.off + li;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pd[k][a] = ok ? pos[3 * i + a] : 0.0;
+            axd[k] = ayd[k] = azd[k] = 0.0;
+            split_df(pd[k][0], xh[k], xl[k]);
+            split_df(pd[k][1], yh[k], yl[k]);
+            split_df(pd[k][2], zh[k], zl[k]);
+            sx[k] = sy[k] = sz[k] = cx[k] = cy[k] = cz[k] = 0.f;
+        }
         const float eps2f = static_cast<float>(S.eps2);
         for (int t0 = 0; t0 < S.n; t0 += BS) {
             const int j = t0 + tid;
@@ -122,53 +132,68 @@ __global__ __launch_bounds__(BS) void k_batch_force(
             if constexpr (MODE == 0) {
                 for (int jj = 0; jj < cnt; ++jj) {
                     const double4 q = s_hi[jj];
-                    const double dx = q.x - px;
-                    const double dy = q.y - py;
-                    const double dz = q.z - pz;
-                    const double r2 = dx * dx + dy * dy + dz * dz + S.eps2;
-                    const double inv = 1.0 / sqrt(r2);
-                    const double s = S.G * q.w * inv * inv * inv;
-                    axd += s * dx;
-                    ayd += s * dy;
-                    azd += s * dz;
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) {
+                        const double dx = q.x - pd[k][0];
+                        const double dy = q.y - pd[k][1];
+                        const double dz = q.z - pd[k][2];
+                        const double r2 = dx * dx + dy * dy + dz * dz + S.eps2;
+                        const double inv = 1.0 / sqrt(r2);
+                        const double s = S.G * q.w * inv * inv * inv;
+                        axd[k] += s * dx;
+                        ayd[k] += s * dy;
+                        azd[k] += s * dz;
+                    }
                 }
             } else {
                 for (int c0 = 0; c0 < cnt; c0 += kChunk) {
                     const int c1 = min(c0 + kChunk, cnt);
-                    float qxp = 0.f, qyp = 0.f, qzp = 0.f;
+                    float qxp[IPT], qyp[IPT], qzp[IPT];
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) qxp[k] = qyp[k] = qzp[k] = 0.f;
 #pragma unroll 4
                     for (int jj = c0; jj < c1; ++jj) {
                         const float4 q = s_hi[jj];
                         const float4 l = s_lo[jj];
-                        const float dx = (q.x - xh) + (l.x - xl);
-                        const float dy = (q.y - yh) + (l.y - yl);
-                        const float dz = (q.z - zh) + (l.z - zl);
-                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
-                        const float inv = __builtin_amdgcn_rsqf(r2);
-                        const float s = q.w * (inv * inv) * inv;
-                        qxp = fmaf(s, dx, qxp);
-                        qyp = fmaf(s, dy, qyp);
-                        qzp = fmaf(s, dz, qzp);
+#pragma unroll
+                        for (int k = 0; k < IPT; ++k) {
+                            const float dx = (q.x - xh[k]) + (l.x - xl[k]);
+                            const float dy = (q.y - yh[k]) + (l.y - yl[k]);
+                            const float dz = (q.z - zh[k]) + (l.z - zl[k]);
+                            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
+                            const float inv = __builtin_amdgcn_rsqf(r2);
+                            const float s = q.w * (inv * inv) * inv;
+                            qxp[k] = fmaf(s, dx, qxp[k]);
+                            qyp[k] = fmaf(s, dy, qyp[k]);
+                            qzp[k] = fmaf(s, dz, qzp[k]);
+                        }
                     }
-                    float e;
-                    two_sum(sx, qxp, sx, e);
-                    cx += e;
-                    two_sum(sy, qyp, sy, e);
-                    cy += e;
-                    two_sum(sz, qzp, sz, e);
-                    cz += e;
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) {
+                        float e;
+                        two_sum(sx[k], qxp[k], sx[k], e);
+                        cx[k] += e;
+                        two_sum(sy[k], qyp[k], sy[k], e);
+                        cy[k] += e;
+                        two_sum(sz[k], qzp[k], sz[k], e);
+                        cz[k] += e;
+                    }
                 }
             }
             __syncthreads();
         }
-        if (ok) {
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int li = it.y + tid + k * BS;
+            if (li >= S.n) continue;
+            const int i = S.off + li;
             double ax, ay, az;
             if constexpr (MODE == 0) {
-                ax = axd; ay = ayd; az = azd;
+                ax = axd[k]; ay = ayd[k]; az = azd[k];
             } else {
-                ax = static_cast<double>(sx) + static_cast<double>(cx);
-                ay = static_cast<double>(sy) + static_cast<double>(cy);
-                az = static_cast<double>(sz) + static_cast<double>(cz);
+                ax = static_cast<double>(sx[k]) + static_cast<double>(cx[k]);
+                ay = static_cast<double>(sy[k]) + static_cast<double>(cy[k]);
+                az = static_cast<double>(sz[k]) + static_cast<double>(cz[k]);
             }
             acc[3 * i] = ax;
             acc[3 * i + 1] = ay;
@@ -259,27 +284,26 @@ __global__ void k_batch_energy_reduce(const double *__restrict__ part,
 }
 
 template <int MODE>
-int launch_force(int bs, int grid, const double *pos, const double *mass, double *acc,
+int launch_force(int tile, int grid, const double *pos, const double *mass, double *acc,
                  double *vel, const int2 *items, int n_items, const SysInfo *sys, int *counter,
                  int kick, hipStream_t st)
 {
-    switch (bs) {
-    case 64:
-        k_batch_force<MODE, 64><<<grid, 64, 0, st>>>(pos, mass, acc, vel, items, n_items, sys,
-                                                     counter, kick);
-        break;
-    case 128:
-        k_batch_force<MODE, 128><<<grid, 128, 0, st>>>(pos, mass, acc, vel, items, n_items, sys,
-                                                       counter, kick);
-        break;
-    case 256:
-        k_batch_force<MODE, 256><<<grid, 256, 0, st>>>(pos, mass, acc, vel, items, n_items, sys,
-                                                       counter, kick);
-        break;
+    // tile = i-particles per work item: one per thread up to 256, then 256
+    // threads with 2 or 4 each
+#define RG_BATCH_LAUNCH(BS, IPT)                                                              \
+    k_batch_force<MODE, BS, IPT><<<grid, BS, 0, st>>>(pos, mass, acc, vel, items, n_items, sys, \
+                                                      counter, kick)
+    switch (tile) {
+    case 64: RG_BATCH_LAUNCH(64, 1); break;
+    case 128: RG_BATCH_LAUNCH(128, 1); break;
+    case 256: RG_BATCH_LAUNCH(256, 1); break;
+    case 512: RG_BATCH_LAUNCH(256, 2); break;
+    case 1024: RG_BATCH_LAUNCH(256, 4); break;
     default:
-        rg::set_error("batch tile must be 64, 128 or 256 (got %d)", bs);
+        rg::set_error("batch tile must be 64, 128, 256, 512 or 1024 (got %d)", tile);
         return RG_ERR_ARG;
     }
+#undef RG_BATCH_LAUNCH
     RG_CHECK_LAUNCH();
     return RG_OK;
 }
@@ -290,7 +314,7 @@ struct rg_batch {
     int nsys = 0;
     int ntot = 0;
     int precision = RG_FP32C;
-    int tile = 128;          // i-particles per work item = threads per workgroup
+    int tile = 128;          // i-particles per work item (threads: min(tile, 256))
     int grid = 0;            // persistent workgroups (0 = 4 per CU)
     int device = 0;
     std::vector<int> off;    // host copy, nsys + 1
@@ -431,8 +455,8 @@ extern "C" int rg_batch_create(int64_t nsys, const int64_t *offsets, const doubl
 extern "C" int rg_batch_set_shape(rg_batch *h, int tile, int grid)
 {
     RG_REQUIRE(h, RG_ERR_ARG, "null handle");
-    RG_REQUIRE(tile == 64 || tile == 128 || tile == 256, RG_ERR_ARG,
-               "tile must be 64, 128 or 256");
+    RG_REQUIRE(tile == 64 || tile == 128 || tile == 256 || tile == 512 || tile == 1024,
+               RG_ERR_ARG, "tile must be 64, 128, 256, 512 or 1024");
     RG_REQUIRE(grid >= 0 && grid <= 65536, RG_ERR_ARG, "grid out of range");
     if (tile != h->tile) {
         h->tile = tile;

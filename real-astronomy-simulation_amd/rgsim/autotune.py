@@ -1,10 +1,11 @@
 """Launch-shape autotuner: times candidate shapes on the caller's actual
 workload with HIP events (torch.cuda.Event on the launch stream) and keeps
-the fastest.  No reference counterpart (the reference uses Taichi's default
-block size, SURVEYed as having no block_dim overrides).
+the fastest.  No reference counterpart (the reference runs Taichi's default
+block sizes; it sets no block_dim).
 
-* ``tune_batch(ens)``  tile (i-particles per work item = threads per
-  workgroup) x persistent grid size of the batched force kernel.
+* ``tune_batch(ens)``  tile (i-particles per work item: one per thread up
+  to 256, then 2 or 4 per thread on 256 threads) x persistent grid size of
+  the batched force kernel.
 * ``tune_direct(pos, mass, ...)``  i-particles per lane x j-slices of the
   direct kernels (FP64 sums are never split, so only i-particles per lane
   are tried for them).
@@ -52,7 +53,8 @@ def _device_key():
     return torch.cuda.get_device_name(torch.cuda.current_device())
 
 
-def tune_batch(ens, tiles=(64, 128, 256), grids_per_cu=(1, 2, 4, 8), reps=2, cache_path=None):
+def tune_batch(ens, tiles=(64, 128, 256, 512, 1024), grids_per_cu=(1, 2, 4, 8), reps=2,
+               cache_path=None):
     key = f"batch|{_device_key()}|{ens.nsys}|{ens.ntot}|{ens.interactions_per_step()}|{ens.precision}"
     cache = _cache_load(cache_path)
     if key in cache:

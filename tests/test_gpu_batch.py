@@ -20,7 +20,7 @@ def unsplit():
     _lib.set_param("direct_split", 0)
 
 
-def small_systems(sizes=(1, 2, 37, 256, 300, 1000, 129), G=1.0):
+def small_systems(sizes=(1, 2, 37, 256, 300, 1000, 129, 2500), G=1.0):
     from rgsim.ic import make_plummer
     out = []
     for k, n in enumerate(sizes):
@@ -32,7 +32,7 @@ def small_systems(sizes=(1, 2, 37, 256, 300, 1000, 129), G=1.0):
 
 
 @pytest.mark.parametrize("precision", ["fp32c", "fp64"])
-@pytest.mark.parametrize("tile", [64, 128, 256])
+@pytest.mark.parametrize("tile", [64, 128, 256, 512, 1024])
 def test_batch_acc_equals_per_system_direct(orc, unsplit, precision, tile):
     from rgsim.batch import Ensemble
     from rgsim.solvers import gravity
@@ -116,6 +116,7 @@ def test_autotuner_picks_a_valid_shape():
     from rgsim.batch import Ensemble
     ens = Ensemble(small_systems())
     best = tune_batch(ens, grids_per_cu=(1, 2), reps=1)
-    assert best["tile"] in (64, 128, 256) and best["ms"] > 0
+    assert best["tile"] in (64, 128, 256, 512, 1024) and best["ms"] > 0
+    assert len(best["trials"]) == 10
     ens.compute_acc()
     assert torch.isfinite(ens.acc).all()
