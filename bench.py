@@ -550,6 +550,18 @@ def copy_calibration(mib=1024, reps=3):
                       "reps": reps}))
 
 
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def host_threads():
     """Host threads for the CPU baselines: OMP_NUM_THREADS when set (the
     GPU box sets its per-GPU CPU share there), else the CPU count, at most 16."""
@@ -581,7 +593,7 @@ def cpu_baseline_direct(n=32768):
                       "sample": f"one FP64 evaluation at N={len(msub)}, {w:.2f} s on {t} thread(s)"}
     orc.set_threads(threads)
     return {"value": out["all"]["value"], "unit": "Ginteractions/s", "cores": threads,
-            "kind": "port", "sample": out["all"]["sample"] + " (the oracle's C/OpenMP "
+            "cpu_model": host_cpu_model(), "kind": "port", "sample": out["all"]["sample"] + " (the oracle's C/OpenMP "
             "restatement of gravity.py:13-26)", "single_thread": out["single_thread"]}
 
 
@@ -616,7 +628,7 @@ def cpu_baseline(steps=10):
     wall1 = run(1, 1)
     orc.set_threads(threads)
     return {"value": s.n * steps / wall / 1e6, "unit": "Mparticle-steps/s",
-            "cores": cores, "kind": "port",
+            "cores": cores, "cpu_model": host_cpu_model(), "kind": "port",
             "sample": f"{steps} KDK steps of the FP64 BH oracle (C/OpenMP restatement of "
                       f"barnes_hut.py + leapfrog.py) on the same N={s.n} disk+bulge, "
                       f"{wall:.1f} s on {cores} threads",
