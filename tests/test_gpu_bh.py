@@ -227,3 +227,41 @@ def test_bh_1m_keys_order_topology_bitexact(orc):
     assert np.array_equal(left, t["left"])
     assert np.array_equal(right, t["right"])
     assert np.array_equal(parent, t["parent"])
+
+
+def test_bh_dropin_config_ic_and_after_200_steps(orc):
+    """The Barnes-Hut drop-in configuration at full size: disk N=100,000 +
+    bulge N=20,000 (seed 1), G 43009.1725, eps 0.05, theta 0.5, 63-bit keys,
+    200 KDK steps at dt 1e-4 through the Engine (engine.py:108-116).  Keys,
+    stable order and topology equal the oracle's on the initial positions
+    and again on the positions after the 200 steps; the force error against
+    the FP64 direct sum stays within verify_bh_galaxy.py:54's 5% mean."""
+    from rgsim.engine import Engine
+    from rgsim.ic import make_disk_bulge
+    from rgsim.solvers import gravity
+    print("bh drop-in config N=120000, 200 steps", flush=True)
+    s = make_disk_bulge(n_disk=100000, n_bulge=20000, seed=1)
+    eng = Engine(softening=0.05, gravity_mode="bh", theta=0.5, precision="fp32")
+    eng.load_state(s)
+
+    def check_tree(pos_np):
+        keys, order, left, right, parent = (t.cpu().numpy() for t in eng._bh.export_tree())
+        t = orc.bh_tree(pos_np, s.mass)
+        assert np.array_equal(order, t["order"])
+        assert np.array_equal(keys.astype(np.uint64), t["keys"][t["order"]])
+        assert np.array_equal(left, t["left"]) and np.array_equal(right, t["right"])
+        assert np.array_equal(parent, t["parent"])
+
+    check_tree(s.pos)
+    for _ in range(200):
+        eng.step(1e-4)
+    # the last tree was built on the positions after the last drift
+    pos_end = eng.pos.cpu().numpy()
+    check_tree(pos_end)
+    ref = torch.empty_like(eng.pos)
+    gravity.compute_acc_direct(eng.pos, eng.mass, ref, s.n, eng.G, 0.05 ** 2, "fp64")
+    r = ref.cpu().numpy()
+    rel = np.linalg.norm(eng.acc.cpu().numpy() - r, axis=1) / np.linalg.norm(r, axis=1)
+    print(f"  after 200 steps: force error mean {rel.mean():.2e} max {rel.max():.2e}", flush=True)
+    assert rel.mean() < 0.05
+    assert eng.step_count == 200
