@@ -142,6 +142,30 @@ def bench_bh(steps, warmup):
     }
 
 
+def bench_bh_fp64(state, dt=1e-4, steps=20):
+    """The BH drop-in with the reference's FP64 arithmetic end to end: FP64
+    aggregation and the FP64 stackless walk (k_walk64, one lane per particle,
+    the reference's criterion and pop order): the plain FP64 GPU treecode the
+    FP32 walk is compared with."""
+    import torch
+
+    from rgsim.engine import Engine
+
+    n = state.n
+    e64 = Engine(softening=0.05, gravity_mode="bh", theta=0.5, precision="fp64")
+    e64.load_state(state)
+    for _ in range(3):
+        e64.step(dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e64.step(dt)
+    torch.cuda.synchronize()
+    w = time.perf_counter() - t0
+    return {"precision": "fp64", "steps": steps, "ms_per_step": 1e3 * w / steps,
+            "mps": n * steps / w / 1e6}
+
+
 def bench_bh30(state, dt=1e-4, steps=50):
     """The BH workload with 30-bit keys (10 bits per axis, the option the
     workload text names; 4 sort passes instead of 8), 50 steps.  Its own
@@ -645,8 +669,8 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="only the BH drop-in workload (no direct/adjoint/batch/1M sections)")
-    ap.add_argument("--sections", default="bh,bh30,direct,adjoint,batch,bh1m,engines,grid",
-                    help="comma-separated subset of bh,bh30,direct,adjoint,batch,bh1m,"
+    ap.add_argument("--sections", default="bh,bh30,bhfp64,direct,adjoint,batch,bh1m,engines,grid",
+                    help="comma-separated subset of bh,bh30,bhfp64,direct,adjoint,batch,bh1m,"
                          "engines,grid")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
@@ -665,7 +689,7 @@ def main():
 
     sections = set(args.sections.split(","))
     if args.no_extra:
-        sections &= {"bh", "bh30"}
+        sections &= {"bh", "bh30", "bhfp64"}
     if args.no_direct:
         sections.discard("direct")
     bh = direct = adjoint = batch = bh1m = engines = grid = None
@@ -673,11 +697,18 @@ def main():
         log("bench: BH disk+bulge N=120000 ...")
         bh = bench_bh(args.steps, args.warmup)
         log(f"bench: BH {bh['mps']:.2f} Mparticle-steps/s, {bh['ms_per_step']:.3f} ms/step")
-        if "bh30" in sections:
+        if "bh30" in sections or "bhfp64" in sections:
             from rgsim.ic import make_disk_bulge
-            bh["morton_30bit"] = bench_bh30(make_disk_bulge(n_disk=100000, n_bulge=20000,
-                                                            seed=1))
+            dbs = make_disk_bulge(n_disk=100000, n_bulge=20000, seed=1)
+        if "bh30" in sections:
+            bh["morton_30bit"] = bench_bh30(dbs)
             log(f"bench: BH 30-bit keys {bh['morton_30bit']['ms_per_step']:.3f} ms/step")
+        if "bhfp64" in sections:
+            f64 = bench_bh_fp64(dbs)
+            f64["speedup_of_fp32_walk"] = f64["ms_per_step"] / bh["ms_per_step"]
+            bh["plain_fp64_gpu"] = f64
+            log(f"bench: BH FP64 walk {f64['ms_per_step']:.3f} ms/step "
+                f"({f64['speedup_of_fp32_walk']:.1f}x slower than the FP32 walk)")
     if "direct" in sections:
         log("bench: direct Plummer N=262144 ...")
         direct = bench_direct()
