@@ -29,6 +29,16 @@ def cases():
     q = rng.normal(size=(700, 3))
     q[:, 2] = 0.0  # flat slab
     out["slab700"] = (q, np.ones(700))
+    # every particle at one point: zero span (scale from 1.0), all keys equal,
+    # the topology from the index tie-break alone, every node of zero size
+    out["coincident300"] = (np.tile([[1.5, -2.0, 0.25]], (300, 1)), rng.uniform(0.5, 1.5, 300))
+    # a quarter of the particles massless, some of them in runs, so whole
+    # subtrees are massless (their centre of mass stays 0)
+    s = make_plummer(800, total_mass=1.0, seed=5, G=1.0)
+    mm = s.mass.copy()
+    mm[rng.permutation(800)[:150]] = 0.0
+    mm[np.argsort(s.pos[:, 0])[:50]] = 0.0
+    out["massless800"] = (s.pos, mm)
     return out
 
 
@@ -71,7 +81,8 @@ def test_standalone_morton_codes(orc):
     assert morton_codes(corner).cpu().numpy().tolist() == [0, 0x7FFFFFFFFFFFFFF8]
 
 
-@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000"])
+@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "coincident300",
+                                  "massless800"])
 def test_aggregation(orc, name):
     pos, mass = CASES[name]
     bh, _ = run_bh(pos, mass)
@@ -85,7 +96,8 @@ def test_aggregation(orc, name):
 
 
 @pytest.mark.parametrize("theta", [0.0, 0.3, 0.5, 0.8])
-@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3"])
+@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3",
+                                  "coincident300", "massless800"])
 def test_fp64_walk_bitexact_given_nodes(orc, name, theta):
     """The FP64 walk visits nodes in the reference's stack order with its
     arithmetic: given the same node data the accelerations are bit-equal."""
@@ -158,7 +170,8 @@ def test_nonfinite_is_flagged():
     assert st["nonfinite"] == 1
 
 
-@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3"])
+@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3",
+                                  "coincident300", "massless800"])
 def test_fp32_walk_visits_match_reference_walk(orc, name):
     """The FP32 preorder scan examines the nodes the reference's stack walk
     pops: exactly 2n-1 per particle at theta 0, and (FP32 rounding of the
@@ -174,7 +187,8 @@ def test_fp32_walk_visits_match_reference_walk(orc, name):
     assert abs(st["visits_total"] - visits.sum()) <= 1e-3 * visits.sum() + 2
 
 
-@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3"])
+@pytest.mark.parametrize("name", ["plummer3000", "dups1025", "diskbulge6000", "n2", "n3",
+                                  "coincident300", "massless800"])
 def test_fp32_walk_modes_agree(name):
     """Every FP32 walk variant examines the same nodes as the one-lane walk;
     the split variants add two FP32 partial sums at the end, the wave walks
