@@ -213,6 +213,27 @@ def phase_rooflines(n, phase_ms, passes=8):
     return out
 
 
+def kernel_issue(profile, kernel, ms, interactions, peak=None):
+    """VALU issue rate of one launch of `kernel`: its wave64 VALU instructions
+    per launch from the committed rocprofv3 SQ_INSTS_VALU summary `profile`
+    over the launch time `ms` measured here, against the FP32 issue peak
+    (or `peak`)."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f).get("kernels", {}).get(kernel)
+    if k is None:
+        return None
+    instr = k["valu_instr_per_launch"]
+    rate = instr / (ms * 1e-3) / 1e9
+    peak = VALU_PEAK_GINST if peak is None else peak
+    return {"kernel": kernel, "valu_instr_per_launch": instr,
+            "valu_instr_per_interaction": instr * 64 / interactions,
+            "achieved": rate, "peak": peak, "unit": "G wave-instructions/s",
+            "frac": rate / peak}
+
+
 def direct_issue(n, prec, ms):
     """VALU issue rate of one direct-sum evaluation: the kernel's wave64 VALU
     instructions per launch (rocprofv3 SQ_INSTS_VALU, committed summary) over
@@ -350,6 +371,10 @@ def bench_adjoint(n=16384, steps=400, ckpt=20):
     out = {"n": n, "steps": steps, "checkpoint_every": ckpt, "precision": "fp32c"}
     out["force_ms"] = timed(lambda: gravity.compute_acc_direct(x, m, acc, n, 1.0, eps2, "fp32c"))
     out["vjp_ms"] = timed(lambda: direct_vjp(x, m, lam, 1.0, eps2, "fp32c"))
+    issue = kernel_issue("k_vjp32c_plummer16384_sq.json", "k_vjp32c", out["vjp_ms"], n * n) \
+        if n == 16384 else None
+    if issue is not None:
+        out["vjp_valu_issue"] = issue
     out["vjp_over_force"] = out["vjp_ms"] / out["force_ms"]
     x0 = torch.from_numpy(s.pos).cuda().requires_grad_(True)
     v0 = torch.from_numpy(s.vel).cuda().requires_grad_(True)
@@ -418,6 +443,13 @@ def bench_batch(steps=100):
     f1 = ens64.energies()
     d32 = np.abs((e1[:, 0] + e1[:, 1] - e0[:, 0] - e0[:, 1]) / (e0[:, 0] + e0[:, 1]))
     d64 = np.abs((f1[:, 0] + f1[:, 1] - f0[:, 0] - f0[:, 1]) / (f0[:, 0] + f0[:, 1]))
+    # the tuned tile -> the kernel instance the graph replays
+    tile = tune.get("tile")
+    kname = {64: "k_batch_force<2, 64, 1>", 128: "k_batch_force<2, 128, 1>",
+             256: "k_batch_force<2, 256, 1>", 512: "k_batch_force<2, 256, 2>",
+             1024: "k_batch_force<2, 256, 4>"}.get(tile)
+    issue = kernel_issue("k_batch_force_512systems_sq.json", kname, ms, inter) \
+        if kname and ens.nsys == 512 else None
     out = {"nsys": ens.nsys, "ntot": ens.ntot, "interactions_per_step": inter,
            "ms_per_step_graph": ms, "Gint_per_s_graph": inter / (ms * 1e-3) / 1e9,
            "ms_per_step_per_system_launches": ps_ms,
@@ -426,6 +458,8 @@ def bench_batch(steps=100):
            "autotune": {k: v for k, v in tune.items() if k != "trials"},
            "energy_drift_fp32c": {"max": float(d32.max()), "median": float(np.median(d32))},
            "energy_drift_fp64": {"max": float(d64.max()), "median": float(np.median(d64))}}
+    if issue is not None:
+        out["valu_issue"] = issue
     log(f"batch: {out['Gint_per_s_graph']:.0f} Gint/s graph vs "
         f"{out['Gint_per_s_per_system_launches']:.0f} per-system; drift fp32c max {d32.max():.1e}")
     return out
