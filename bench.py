@@ -355,11 +355,15 @@ def bench_adjoint(n=16384, steps=400, ckpt=20):
     lam = torch.randn_like(x)
     eps2 = 0.01 ** 2
     acc = torch.empty_like(x)
-    reps = 20
+    reps = 50
 
     def timed(fn):
-        fn()
-        torch.cuda.synchronize()
+        # warm up for at least 50 ms of calls, so that the timed calls run at
+        # the clocks a sustained run sees
+        t_end = time.perf_counter() + 0.05
+        while time.perf_counter() < t_end:
+            fn()
+            torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
