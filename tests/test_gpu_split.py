@@ -181,20 +181,43 @@ def test_direct_autotuner_keeps_results_in_tolerance(orc):
         _lib.set_param("direct_split", 0)
 
 
-@pytest.mark.parametrize("n,precision,tol_max,tol_rms", [
-    (65536, "fp32c", 1e-6, 1e-7), (70001, "fp32c", 1e-6, 1e-7), (131073, "fp32", 2e-3, 1e-4)])
-def test_default_shape_large_n(orc, n, precision, tol_max, tol_rms):
+def term_magnitude_sums(pos, mass, eps2, idx):
+    """sum_j |m_j d_ij| / (|d_ij|^2 + eps2)^(3/2) (G = 1) for the particles idx."""
+    out = np.empty(len(idx))
+    for k, i in enumerate(idx):
+        d = pos - pos[i]
+        q = (d * d).sum(1)
+        r2 = q + eps2
+        out[k] = (mass * np.sqrt(q) / (r2 * np.sqrt(r2))).sum()
+    return out
+
+
+@pytest.mark.parametrize("n,precision,seed,tol_max,tol_rms,tol_cond", [
+    (65536, "fp32c", 3, 1e-6, 1e-7, 2e-7), (65536, "fp32c", 0, 1e-5, 1e-7, 2e-7),
+    (70001, "fp32c", 3, 1e-6, 1e-7, 2e-7), (131073, "fp32", 3, 2e-3, 1e-4, 5e-5)])
+def test_default_shape_large_n(orc, n, precision, seed, tol_max, tol_rms, tol_cond):
     """The default launch shape at large n (FP32C four i-particles per lane
     and 64 slices from n = 65,536; FP32 two per lane from 131,072), ragged
-    sizes included, against the oracle: FP32C within the 1e-6 maximum
-    relative error the precision form is held to."""
+    sizes included, against the oracle.  The maximum relative error is set
+    by particles whose forces nearly cancel, so it depends on the
+    realization (seed 3: within the 1e-6 the precision form is held to;
+    seed 0 puts it at 4e-6 on a particle with |a| 1/40 of the median).  The
+    error over the sum of the term magnitudes, on a 128-particle sample plus
+    the worst particle, does not: FP32C stays below 2e-7 in both."""
     from rgsim.solvers import gravity
-    print(f"default shape {precision} n={n}", flush=True)
-    s = plummer(n, seed=3)
+    print(f"default shape {precision} n={n} seed={seed}", flush=True)
+    s = plummer(n, seed=seed)
     ref = orc.direct_acc(s.pos, s.mass, 1.0, 0.005 ** 2)
     p, m = dev(s.pos), dev(s.mass)
     acc = torch.empty_like(p)
     gravity.compute_acc_direct(p, m, acc, n, 1.0, 0.005 ** 2, precision)
-    rel = rel_rows(acc.cpu().numpy(), ref)
-    print(f"  max {rel.max():.2e} rms {np.sqrt(np.mean(rel ** 2)):.2e}", flush=True)
+    got = acc.cpu().numpy()
+    rel = rel_rows(got, ref)
+    idx = np.concatenate([np.random.default_rng(5).choice(n, 128, replace=False),
+                          [int(np.argmax(rel))]])
+    cond = np.linalg.norm(got[idx] - ref[idx], axis=1) / term_magnitude_sums(
+        s.pos, s.mass, 0.005 ** 2, idx)
+    print(f"  max {rel.max():.2e} rms {np.sqrt(np.mean(rel ** 2)):.2e} "
+          f"cond {cond.max():.2e}", flush=True)
     assert rel.max() < tol_max and np.sqrt(np.mean(rel ** 2)) < tol_rms
+    assert cond.max() < tol_cond
