@@ -46,9 +46,21 @@ constexpr int kIpt2From = 131072;
 // wave per 64 i-particles leaves most SIMDs idle there.  Measured on one
 // MI355X (ms per evaluation, one wave / 8 / 16 waves per 64 i-particles):
 // n = 4,096 0.512 / 0.192 / 0.155; 16,384 2.05 / 0.780 / 0.624; 20,000
-// 2.50 / 1.44 / 1.50; 32,768 4.15 / 2.41 / 2.47; 65,536 8.23 / 9.02 / 9.88.
+// 2.50 / 1.44 / 1.50; 32,768 4.15 / 2.41 / 2.47.  Above, the one-wave
+// kernel needs at least two waves on every SIMD, evenly spread, to beat the
+// 8-wave pipeline (about 570 G interactions/s from n = 49,152); it reaches
+// 620 with two waves per SIMD and 657 with four.  Measured (one wave / 8
+// waves, ms): 49,152 6.16 / 4.60; 65,536 (one wave per SIMD) 8.20 / 7.89;
+// 81,920 (1.25) 17.5 / 12.1; 98,304 (1.5) 21.0 / 17.1; 110,000 (1.68, fill
+// 0.84) 23.4 / 22.1; 120,000 (fill 0.92) 25.5 / 27.8; 131,072 (2) 27.8 /
+// 29.7; 163,840 (2.5, fill 0.83) 51.5 / 46.3; 196,608 (3) 60.4 / 67.1;
+// 262,144 (4) 104.6 / 118.9.
 constexpr int kPipe16Upto = 16384;
 constexpr int kPipeBelow = 49152;
+// one-wave waves on the busiest SIMD / the least even fill at which the
+// one-wave kernel is used
+constexpr int kOneWaveMinPerSimd = 2;
+constexpr double kOneWaveMinFill = 0.9;
 
 int g_direct_ipt = 0;         // 0 = heuristic; else i-particles per thread (1, 2, 4)
 int g_direct_split = 0;       // 0 = by n; else j-slices per i-block (FP32/FP32C)
@@ -504,13 +516,37 @@ int launch_direct(const double *pos, const double *mass, double *acc, double *ve
     return RG_OK;
 }
 
+// SIMDs of the current device (4 per CU), cached per device.
+int device_simds()
+{
+    static int simds[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+    if (simds[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess || cus <= 0)
+            return 1024;
+        simds[dev] = 4 * cus;
+    }
+    return simds[dev];
+}
+
 // Waves per 64 i-particles of the FP64 pipeline (1: the one-wave kernel).
 int fp64_pipe_waves(int n)
 {
     if (g_fp64_pipe) return g_fp64_pipe;
     if (g_direct_ipt) return 1;  // an explicit i-per-lane choice picks k_direct<0>
     if (n <= kPipe16Upto) return 16;
-    return n < kPipeBelow ? 8 : 1;
+    if (n < kPipeBelow) return 8;
+    // the one-wave kernel: waves on the busiest SIMD and how evenly the
+    // others are filled
+    const long long waves = (static_cast<long long>(n) + 63) / 64;
+    const long long simds = device_simds();
+    const long long k = (waves + simds - 1) / simds;
+    const double fill = static_cast<double>(waves) / static_cast<double>(k * simds);
+    if (k < kOneWaveMinPerSimd || (k < 4 && fill < kOneWaveMinFill)) return 8;
+    return 1;
 }
 
 int launch_direct64_pipe(int w, const double *pos, const double *mass, double *acc, double *vel,

@@ -47,6 +47,22 @@ def test_pipe_bitexact_vs_oracle(orc, pipe, waves, n):
     assert np.array_equal(got, ref), np.abs(got - ref).max()
 
 
+@pytest.mark.parametrize("n", [65536, 81921, 120000, 131072])
+def test_default_choice_bitexact_large_n(orc, n):
+    """At these sizes the library picks the 8-wave pipeline (fewer than two
+    one-wave waves per SIMD, or an uneven spread: 65,536, 81,921) or the
+    one-wave kernel (120,000, 131,072) by SIMD occupancy; either gives the
+    oracle's doubles."""
+    from rgsim.solvers import gravity
+    print(f"fp64 default choice n={n}", flush=True)
+    s = plummer(n, seed=3)
+    ref = orc.direct_acc(s.pos, s.mass, G_KPC, 0.05 ** 2)
+    p, m = dev(s.pos), dev(s.mass)
+    acc = torch.empty_like(p)
+    gravity.compute_acc_direct(p, m, acc, n, G_KPC, 0.05 ** 2, "fp64")
+    assert np.array_equal(acc.cpu().numpy(), ref)
+
+
 @pytest.mark.parametrize("waves", [4, 8, 16])
 def test_pipe_epilogues_bitexact(orc, pipe, waves):
     from rgsim.solvers import gravity
