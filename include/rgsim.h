@@ -58,8 +58,10 @@ const char *rg_last_error(void);
  *                 and n: FP64 1; FP32C 4 from n = 65,536; FP32 2 from
  *                 n = 131,072; else 1), 1, 2 or 4
  *   "fp64_pipe"   waves per 64 i-particles of the FP64 direct sums: 0 (by n:
- *                 16 up to n = 16,384, 8 below 49,152, else 1), 1 (one wave
- *                 per 64 i-particles), 4, 8 or 16.  Every setting gives the
+ *                 16 up to n = 16,384, 8 below 49,152; above, 1 when that
+ *                 puts at least two waves on every SIMD, filled to 90% of the
+ *                 busiest below four per SIMD, else 8), 1 (one wave per 64
+ *                 i-particles), 4, 8 or 16.  Every setting gives the
  *                 same doubles (the reference's j order); with more than one
  *                 wave, the waves compute pair terms in parallel and three
  *                 waves add them in j order
