@@ -547,6 +547,16 @@ def bench_bh_1m(steps=5):
            "walk_GBs_algorithmic": walk_bytes / (walk_ms * 1e-3) / 1e9,
            "walk_frac_of_hbm_peak": walk_bytes / (walk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "force_err_vs_direct": err, "force_err_reference": "FP32-compensated direct sum"}
+    # bytes past the L2 per walk launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    # copy-scaled, committed summary) over the measured walk time
+    pmc = os.path.join(ROOT, "profiles", "k_walk32_chunked4_true_diskbulge1M_pmc.json")
+    if os.path.exists(pmc) and st["walk_mode"] == 7 and st["walk_waves"] == 4:
+        with open(pmc) as f:
+            rec = json.load(f)
+        if rec.get("n") == n:
+            gbs = rec["hbm_bytes_per_launch"] / (walk_ms * 1e-3) / 1e9
+            out["walk_traffic"] = {"bytes_per_launch": rec["hbm_bytes_per_launch"],
+                                   "GB_per_s": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
     log(f"bh 1M: {out['ms_per_step']:.2f} ms/step, walk {walk_ms:.2f} ms "
         f"({out['walk_frac_of_hbm_peak']:.2f} of HBM peak)")
     return out
