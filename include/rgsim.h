@@ -69,6 +69,10 @@ const char *rg_last_error(void);
  *                 each tile's index from its workgroup when all tiles fit on
  *                 the GPU at once (<= 256), else from an atomic ticket; 1
  *                 always from the ticket.  Same results either way
+ *   "sort_items"  keys per thread in a radix-sort tile, for BH and SPH-grid
+ *                 handles created afterwards: 0 (default: 4 below 524,288
+ *                 keys, 8 below 786,432, else 16), 4, 8 or 16.  Same results
+ *                 either way
  *   "direct_split" j-slices per i-block of the FP32/FP32C direct sums and of
  *                 rg_direct_vjp: 0 (default: 32 with one i-particle per
  *                 lane, 64 with more, fewer when slices x n would pass 2^24)

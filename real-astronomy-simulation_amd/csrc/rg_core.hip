@@ -23,6 +23,7 @@ std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
 namespace rg {
 
 int g_sort_ticket = 0;
+int g_sort_items = 0;
 
 void set_error(const char *fmt, ...)
 {
@@ -75,6 +76,12 @@ extern "C" int rg_set_param(const char *name, int64_t value)
     if (std::strcmp(name, "sort_ticket") == 0) {
         RG_REQUIRE(value == 0 || value == 1, RG_ERR_ARG, "sort_ticket must be 0 or 1");
         rg::g_sort_ticket = static_cast<int>(value);
+        return RG_OK;
+    }
+    if (std::strcmp(name, "sort_items") == 0) {
+        RG_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16, RG_ERR_ARG,
+                   "sort_items must be 0, 4, 8 or 16");
+        rg::g_sort_items = static_cast<int>(value);
         return RG_OK;
     }
     if (std::strcmp(name, "fp32c_d64") == 0) return rg::rg_direct_set_fp32c_d64_impl(value);

@@ -29,6 +29,10 @@ int direct_slices(int n, int ipt, int *jlen);
 // may not all be resident at once (0, the default).
 extern int g_sort_ticket;
 
+// Keys per thread in a radix-sort tile for handles created from now on: 0
+// (by n), 4, 8 or 16.
+extern int g_sort_items;
+
 // Tunables behind rg_set_param.
 int rg_direct_set_ipt_impl(int64_t v);
 int rg_direct_set_split_impl(int64_t v);

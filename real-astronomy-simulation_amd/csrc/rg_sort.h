@@ -20,11 +20,21 @@ constexpr int kSortBS = 256;
 // switch is where 8-per-thread tiles would no longer cover the chip's 256
 // CUs.  Measured for the whole Morton phase (bounding box, keys, 8 passes):
 // 0.113 ms with 4 per thread against 0.122 ms with 8 at 120k keys, and
-// 0.339 against 0.26 ms at 10^6 keys.
+// 0.339 against 0.26 ms at 10^6 keys.  16 per thread from the size where
+// 8-per-thread tiles would need 1.5 workgroups per CU: sort phase per BH
+// step 0.193 against 0.166 ms with 8 at 524,288 keys, 0.238 against 0.265
+// at 10^6, 0.384 against 0.436 at 2 x 10^6.
 constexpr int kSortItemsSmall = 4;
 constexpr int kSortItemsLarge = 8;
+constexpr int kSortItemsHuge = 16;
 constexpr int64_t kSortLargeFrom = 256ll * kSortBS * kSortItemsLarge;  // 524,288 keys
-inline int sort_items_for(int64_t n) { return n < kSortLargeFrom ? kSortItemsSmall : kSortItemsLarge; }
+constexpr int64_t kSortHugeFrom = 384ll * kSortBS * kSortItemsLarge;   // 786,432 keys
+inline int sort_items_for(int64_t n)
+{
+    if (rg::g_sort_items) return rg::g_sort_items;
+    if (n < kSortLargeFrom) return kSortItemsSmall;
+    return n < kSortHugeFrom ? kSortItemsLarge : kSortItemsHuge;
+}
 inline int sort_tiles(int64_t n, int items)
 {
     const int64_t tile = static_cast<int64_t>(kSortBS) * items;
@@ -265,6 +275,9 @@ inline void launch_onesweep(int items, int nblk, hipStream_t st, const uint64_t 
     if (items == kSortItemsSmall) {
         if (ticket) RG_ONESWEEP(kSortItemsSmall, true);
         else RG_ONESWEEP(kSortItemsSmall, false);
+    } else if (items == kSortItemsHuge) {
+        if (ticket) RG_ONESWEEP(kSortItemsHuge, true);
+        else RG_ONESWEEP(kSortItemsHuge, false);
     } else {
         if (ticket) RG_ONESWEEP(kSortItemsLarge, true);
         else RG_ONESWEEP(kSortItemsLarge, false);
