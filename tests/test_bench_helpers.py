@@ -1,0 +1,56 @@
+"""The bench's measurement helpers (CPU): the sum of term magnitudes behind
+the condition-normalised force error against a pairwise loop, and the VALU
+issue figures read from the committed rocprofv3 summaries."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def bench():
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_term_magnitude_sums_pairwise(bench):
+    rng = np.random.default_rng(2)
+    pos = rng.normal(size=(70, 3))
+    mass = rng.uniform(0.5, 1.5, 70)
+    idx = np.array([0, 5, 69, 33])
+    got = bench.term_magnitude_sums(pos, mass, 2.5, 1e-3, idx, rows=3)
+    for k, i in enumerate(idx):
+        ref = 0.0
+        for j in range(70):
+            d = pos[j] - pos[i]
+            r2 = d @ d + 1e-3
+            ref += 2.5 * mass[j] * np.sqrt(d @ d) / r2 ** 1.5
+        assert got[k] == pytest.approx(ref, rel=1e-13)
+
+
+def test_cond_err_zero_and_scale(bench):
+    ref = np.ones((4, 3))
+    a = ref.copy()
+    a[2, 0] += 3.0
+    e = bench.cond_err(a, ref, np.array([1, 2]), np.array([10.0, 30.0]))
+    assert e["max"] == pytest.approx(0.1) and e["sample"] == 2
+
+
+def test_issue_figures_from_profiles(bench):
+    # FP32C direct sum at N=262,144: about 20 VALU instructions per pair
+    d = bench.direct_issue(262144, "fp32c", 25.0)
+    assert d is not None and 19 < d["valu_instr_per_interaction"] < 21
+    assert d["peak"] == pytest.approx(1228.8)
+    assert d["achieved"] == pytest.approx(d["valu_instr_per_launch"] / 25e-3 / 1e9)
+    f = bench.direct_issue(262144, "fp64", 104.0)
+    assert f["peak"] == pytest.approx(614.4) and 45 < f["valu_instr_per_interaction"] < 52
+    assert bench.direct_issue(1000, "fp32c", 1.0) is None
+    b = bench.kernel_issue("k_batch_force_512systems_sq.json", "k_batch_force<2, 256, 1>",
+                           6.0, 15509199223)
+    assert b is not None and 19 < b["valu_instr_per_interaction"] < 22
+    assert bench.kernel_issue("k_batch_force_512systems_sq.json", "no_such_kernel", 1.0, 1) is None
