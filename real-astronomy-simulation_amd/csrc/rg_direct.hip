@@ -16,6 +16,7 @@
 // expression order with IEEE 1.0/sqrt, so each particle's FP64 result is the
 // same double the sequential-j CPU evaluation gives.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 
 #include "rg_internal.h"
@@ -516,20 +517,23 @@ int launch_direct(const double *pos, const double *mass, double *acc, double *ve
     return RG_OK;
 }
 
-// SIMDs of the current device (4 per CU), cached per device.
+// SIMDs of the current device (4 per CU), cached per device; threads that
+// race on the first query store the same value.
 int device_simds()
 {
-    static int simds[64] = {};
+    static std::atomic<int> simds[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
-    if (simds[dev] == 0) {
+    int v = simds[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
                 hipSuccess || cus <= 0)
             return 1024;
-        simds[dev] = 4 * cus;
+        v = 4 * cus;
+        simds[dev].store(v, std::memory_order_relaxed);
     }
-    return simds[dev];
+    return v;
 }
 
 // Waves per 64 i-particles of the FP64 pipeline (1: the one-wave kernel).
