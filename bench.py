@@ -48,6 +48,8 @@ WALK_KERNELS = {0: "k_walk32", 1: "k_walk32_pair", 2: "k_walk32_wave", 3: "k_wal
                 4: "k_walk32_wave_split<2>", 5: "k_walk32_wave_split<4>",
                 6: "k_walk32_chunked<{w}>", 7: "k_walk32_chunked<{w}, true>", -1: "k_walk64"}
 WALK_PARTICLE_BYTES = 16 + 4 + 24  # own leaf record + order index + FP64 acceleration
+WALK_NODE_BYTES = 16 + 8           # FP32 record pair (com, G m) + (threshold, skip), per node
+WALK_IO_BYTES = 4 + 24             # order index read + FP64 acceleration written, per particle
 
 
 def log(msg):
@@ -150,6 +152,33 @@ def bench_bh(steps, warmup):
                              "unit": "G wave-instructions/s", "frac": rate / VALU_PEAK_GINST,
                              "kernel": kernel,
                              "valu_instr_per_launch": rec["valu_instr_per_launch"]}
+    # The walk against HBM: the bytes it must move at least (every FP32 node
+    # record once, 24 B, and per particle its order index and FP64
+    # acceleration, 28 B) and the bytes it did move past the L2 (rocprofv3
+    # FETCH_SIZE + WRITE_SIZE, copy-scaled).  The per-visit record figure
+    # (effective_GBs) counts LDS, register-window and cache hits too, so it
+    # is a reuse rate, not an HBM fraction.
+    compulsory = (2 * n - 1) * WALK_NODE_BYTES + n * WALK_IO_BYTES
+    hbm = {"compulsory_bytes_per_launch": compulsory,
+           "compulsory_GBs": compulsory / (walk_ms * 1e-3) / 1e9,
+           "traffic_bytes_per_launch": traffic,
+           "traffic_GBs": traffic / (walk_ms * 1e-3) / 1e9 if traffic else None,
+           "traffic_frac_of_hbm_peak": (traffic / (walk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                        if traffic else None),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    if roofline_valu is not None:
+        # the bound the counters show: VALU issue (the walk is far from HBM
+        # peak at every size; its waves wait on dependent LDS and scalar
+        # results most of their lifetime)
+        roofline = dict(roofline_valu)
+        roofline.update({"traffic": traffic, "avg_launch_ms": walk_ms,
+                         "effective_GBs": achieved,
+                         "algorithmic_record_bytes_per_launch": walk_bytes})
+    else:
+        roofline = {"bound": "hbm", "achieved": hbm["compulsory_GBs"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": hbm["compulsory_GBs"] / HBM_PEAK_GBS,
+                    "traffic": traffic, "kernel": kernel, "avg_launch_ms": walk_ms,
+                    "effective_GBs": achieved}
     return {
         "n": n, "wall_s": wall, "ms_per_step": 1e3 * wall / steps,
         "mps": n * steps / wall / 1e6,
@@ -163,11 +192,8 @@ def bench_bh(steps, warmup):
         "phase_hbm": phase_rooflines(n, {"sort": tm["sort_ms"] / max(tm["count"], 1),
                                          "build": tm["build_ms"] / max(tm["count"], 1),
                                          "aggregate": tm["aggregate_ms"] / max(tm["count"], 1)}),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kernel, "algorithmic_bytes_per_launch": walk_bytes,
-                     "avg_launch_ms": walk_ms},
-        "roofline_valu": roofline_valu,
+        "roofline": roofline,
+        "walk_hbm": hbm,
         "energy": None,
     }
 
@@ -509,8 +535,12 @@ def bench_bh_1m(steps=5):
     eng = Engine(softening=0.05, gravity_mode="bh", theta=0.5, precision="fp32")
     eng.load_state(s)
     ref = torch.empty_like(eng.pos)
-    gravity.compute_acc_direct(eng.pos, eng.mass, ref, n, eng.G, 0.05 ** 2, "fp32c")
+    t0 = time.perf_counter()
+    gravity.compute_acc_direct(eng.pos, eng.mass, ref, n, eng.G, 0.05 ** 2, "fp64")
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t0
     err = rel_err(eng.acc.cpu().numpy(), ref.cpu().numpy())
+    del ref
     eng.step(1e-4)
     torch.cuda.synchronize()
     eng._bh.set_timing(True)
@@ -544,9 +574,11 @@ def bench_bh_1m(steps=5):
            "phase_hbm": phase_rooflines(n, phases),
            "kick_drift": {"ms": kd_ms, "GB_per_s": kd_gbs, "frac_of_hbm_peak": kd_gbs / HBM_PEAK_GBS},
            "visits_per_particle": st["visits_total"] / n,
-           "walk_GBs_algorithmic": walk_bytes / (walk_ms * 1e-3) / 1e9,
-           "walk_frac_of_hbm_peak": walk_bytes / (walk_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-           "force_err_vs_direct": err, "force_err_reference": "FP32-compensated direct sum"}
+           "walk_effective_GBs": walk_bytes / (walk_ms * 1e-3) / 1e9,
+           "walk_compulsory_GBs": ((2 * n - 1) * WALK_NODE_BYTES + n * WALK_IO_BYTES)
+                                  / (walk_ms * 1e-3) / 1e9,
+           "force_err_vs_direct": err,
+           "force_err_reference": f"FP64 direct sum (the reference's arithmetic), {ref_s:.2f} s"}
     # bytes past the L2 per walk launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
     # copy-scaled, committed summary) over the measured walk time
     pmc = os.path.join(ROOT, "profiles", "k_walk32_chunked4_true_diskbulge1M_pmc.json")
@@ -682,24 +714,53 @@ def copy_calibration(mib=1024, reps=3):
 
 
 def host_cpu_model():
-    """The host CPU's model name (/proc/cpuinfo), or None."""
+    """The host CPU's model name with its family/model/stepping numbers
+    (/proc/cpuinfo), or None."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break  # first processor only
+                k, _, v = line.partition(":")
+                fields[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    name = fields.get("model name")
+    if name is None:
+        return None
+    ids = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    return f"{name} ({', '.join(ids)})" if ids else name
+
+
+def host_cpus():
+    """(cores this process may run on, cores the host has)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count() or usable
 
 
 def host_threads():
-    """Host threads for the CPU baselines: OMP_NUM_THREADS when set (the
-    GPU box sets its per-GPU CPU share there), else the CPU count, at most 16."""
+    """Host threads for the CPU baselines: every core this process may run
+    on (its affinity mask), unless OMP_NUM_THREADS is set lower -- the GPU
+    box sets its per-GPU CPU share there."""
+    usable, _ = host_cpus()
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
-        return int(env)
-    return max(1, min(os.cpu_count() or 1, 16))
+        return min(int(env), usable)
+    return usable
+
+
+def host_cpu_info(threads):
+    usable, nproc = host_cpus()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return {"cpu_model": host_cpu_model(), "nproc": nproc, "affinity_cores": usable,
+            "threads_used": threads,
+            "threads_from": ("OMP_NUM_THREADS (the per-GPU CPU share)"
+                             if env.isdigit() and int(env) > 0 and int(env) < usable
+                             else "affinity mask")}
 
 
 def cpu_baseline_direct(n=32768):
@@ -724,7 +785,7 @@ def cpu_baseline_direct(n=32768):
                       "sample": f"one FP64 evaluation at N={len(msub)}, {w:.2f} s on {t} thread(s)"}
     orc.set_threads(threads)
     return {"value": out["all"]["value"], "unit": "Ginteractions/s", "cores": threads,
-            "cpu_model": host_cpu_model(), "kind": "port", "sample": out["all"]["sample"] + " (the oracle's C/OpenMP "
+            **host_cpu_info(threads), "kind": "port", "sample": out["all"]["sample"] + " (the oracle's C/OpenMP "
             "restatement of gravity.py:13-26)", "single_thread": out["single_thread"]}
 
 
@@ -759,7 +820,7 @@ def cpu_baseline(steps=10):
     wall1 = run(1, 1)
     orc.set_threads(threads)
     return {"value": s.n * steps / wall / 1e6, "unit": "Mparticle-steps/s",
-            "cores": cores, "cpu_model": host_cpu_model(), "kind": "port",
+            "cores": cores, **host_cpu_info(cores), "kind": "port",
             "sample": f"{steps} KDK steps of the FP64 BH oracle (C/OpenMP restatement of "
                       f"barnes_hut.py + leapfrog.py) on the same N={s.n} disk+bulge, "
                       f"{wall:.1f} s on {cores} threads",
@@ -860,7 +921,11 @@ def main():
         "ms_per_step": bh["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": (bh["plain_fp64_gpu"]["ms_per_step"] / bh["ms_per_step"]
+                        if bh.get("plain_fp64_gpu") else None),
+        "vs_baseline_basis": "the plain FP64 GPU treecode (the reference's arithmetic, "
+                             "criterion and pop order: FP64 aggregation and walk) on the same "
+                             "workload in the same run, as the metric names",
         "dtype": "f32",
         "data": "synthetic (seeded disk+bulge and Plummer generators)",
         "config": {"workload": "BH drop-in: sech2 exp disk N=100000 + Plummer bulge N=20000, "
@@ -869,9 +934,8 @@ def main():
                    "walk_precision": "fp32", "aggregation_precision": "fp64",
                    "parallelism": "single GPU"},
         "roofline": bh["roofline"],
-        "roofline_valu": bh["roofline_valu"],
         "cpu_baseline": cpu,
-        "bh": {k: v for k, v in bh.items() if k not in ("roofline", "roofline_valu")},
+        "bh": {k: v for k, v in bh.items() if k != "roofline"},
         "direct_plummer_262144": direct,
         "adjoint_plummer_16384": adjoint,
         "batch_512_systems": batch,

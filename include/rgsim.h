@@ -69,6 +69,10 @@ const char *rg_last_error(void);
  *                 each tile's index from its workgroup when all tiles fit on
  *                 the GPU at once (<= 256), else from an atomic ticket; 1
  *                 always from the ticket.  Same results either way
+ *   "sort_spin_limit" re-polls a decoupled look-back wait (radix sort,
+ *                 treelet ranking) makes before it gives up and flags the
+ *                 failure: -1 restores the default 2^20, 0 gives up at once
+ *                 (to test the failure path; every kernel still exits)
  *   "sort_items"  keys per thread in a radix-sort tile, for BH and SPH-grid
  *                 handles created afterwards: 0 (default: 4 below 524,288
  *                 keys, 8 below 786,432, else 16), 4, 8 or 16.  Same results
@@ -193,8 +197,18 @@ typedef struct rg_bh rg_bh;
 /* BarnesHut(n, theta, softening) (core/solvers/barnes_hut.py:223-240).
  * n >= 2.  G is explicit (the reference hard-codes core.units.G).
  * precision RG_FP64 walks FP64 node records with the reference's exact
- * criterion and operation order; RG_FP32/RG_FP32C walk FP32 node records
- * (aggregation stays FP64).  Allocates all scratch on the current device. */
+ * criterion and operation order; RG_FP32 walks FP32 node records
+ * (aggregation stays FP64).  RG_FP32C is rejected (RG_ERR_ARG): the walk
+ * has no compensated form.  Allocates all scratch on the current device.
+ *
+ * Failure reporting of the compute calls below: the radix sort's and the
+ * treelet ranking's look-back waits are bounded ("sort_spin_limit").  When
+ * one gives up, every kernel of that compute after the sort returns without
+ * touching acc (or vel), and the device sets a pinned host word of the
+ * handle.  The compute calls are asynchronous, so the failure is returned
+ * (RG_ERR_CAPACITY, a radix-sort message) by the next rg_bh_compute*,
+ * rg_bh_step or rg_bh_export_* call on the handle, or by rg_bh_stats, which
+ * synchronises first; it is reported once. */
 int rg_bh_create(int64_t n, double theta, double eps, double G, int morton_bits,
                  int precision, rg_bh **out);
 
@@ -250,7 +264,10 @@ typedef struct {
     int64_t walk_waves;     /* waves per group of 64 particles in it (0: one lane each) */
 } rg_bh_stats_t;
 
-/* Statistics of the last compute.  Synchronises that compute's stream. */
+/* Statistics of the last compute.  Synchronises that compute's stream.
+ * Returns RG_ERR_CAPACITY (with *out filled) if that compute aborted (a
+ * look-back wait gave up) or its walk or preorder pass ran out of bounds (a
+ * corrupt tree). */
 int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out);
 
 /* Per-phase timing with HIP events recorded on the compute's stream (no

@@ -99,11 +99,24 @@ __device__ __forceinline__ void block_box_reduce(double v[7], double (*s_w)[7])
     }
 }
 
+// Statistics words of a compute (device, 8 x u64): [0] nodes visited by the
+// walk, [1] the most one particle visited, [2] non-finite coordinates,
+// [3] look-back waits that gave up (radix sort, treelet ranking: the order
+// and tree are then invalid), [4] walks or preorder chains that ran out of
+// bounds (a corrupt tree).
+constexpr int kStatWords = 8;
+
+// A look-back wait of this compute gave up (stats[3]): every kernel after
+// the sort returns at once, so nothing indexes through the invalid order.
+__device__ __forceinline__ bool tree_aborted(const unsigned long long *__restrict__ stats)
+{
+    return stats[3] != 0ull;
+}
+
 // Resets, by block 0 of the first pass of a compute, for the Morton pass
 // and the sort and tree passes after it: the digit histograms, the sort's
 // tile tickets, the tree's counters (rank tickets at [2], treelets at [3])
-// and the visit and look-back-failure statistics (stats[0], [1], [3]; [2] is
-// written by the Morton pass).
+// and the statistics (all but [2], which the Morton pass writes).
 __device__ __forceinline__ void reset_counters(unsigned long long *__restrict__ stats,
                                                unsigned *__restrict__ ghist,
                                                int *__restrict__ tickets,
@@ -118,6 +131,7 @@ __device__ __forceinline__ void reset_counters(unsigned long long *__restrict__ 
         stats[0] = 0ull;
         stats[1] = 0ull;
         stats[3] = 0ull;
+        stats[4] = 0ull;
     }
 }
 
@@ -352,8 +366,10 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
                         int *__restrict__ right, int *__restrict__ parent,
                         int *__restrict__ up, int2 *__restrict__ range,
                         int *__restrict__ left_at_split, int *__restrict__ tl_of,
-                        unsigned *__restrict__ rank_lb, int rank_tiles)
+                        unsigned *__restrict__ rank_lb, int rank_tiles,
+                        const unsigned long long *__restrict__ stats)
 {
+    if (tree_aborted(stats)) return;
     const int64_t nn = n;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rank_tiles;
          k += gridDim.x * blockDim.x)
@@ -408,7 +424,7 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
 // Right-first DFS preorder index of every node (the order the reference's
 // stack walk pops nodes in): pre(v) = 2 (n-1-hi(v)) + (right turns on the
 // root-to-v path), hi = last leaf of v's range.  The subtree of v then
-// occupies [pre(v), pre(v) + 2 (hi-lo+1) - 1).  stats[3] flags a chain
+// occupies [pre(v), pre(v) + 2 (hi-lo+1) - 1).  stats[4] flags a chain
 // longer than any valid tree (corrupt links).
 constexpr int kMaxDepth = 128;  // > 63 + 31 + 1, the deepest tree the tie-break allows
 
@@ -426,7 +442,7 @@ __device__ void preorder_part(const int *__restrict__ up, const int2 *__restrict
             u = x >> 1;
             ++steps;
         }
-        if (u != n) atomicAdd(&stats[3], 1ull);
+        if (u != n) atomicAdd(&stats[4], 1ull);
         pre[v] = 2 * (n - 1 - hi) + r;
     }
 }
@@ -453,6 +469,8 @@ struct WalkOut {
     const int *pre;
     double G, inv_theta2;
     int theta_zero;
+    const unsigned long long *stats;  // tree_aborted() is read from it
+    unsigned long long *host_flag;    // pinned host word: set when the compute aborted
 };
 
 // pslot: the node's preorder slot (FP32 records; the caller has it at hand).
@@ -505,7 +523,7 @@ constexpr int kRankTile = 256 * kRankItems;  // leaves per treelet-rank tile
 __device__ void treelet_rank_tile(const int *__restrict__ tl_of, int n, int *__restrict__ trank,
                                   int *__restrict__ tsorted, unsigned *__restrict__ lb,
                                   int *__restrict__ counts, int ntiles,
-                                  unsigned long long *__restrict__ stats)
+                                  unsigned long long *__restrict__ stats, int spin_limit)
 {
     __shared__ int s_tile;
     __shared__ unsigned s_wsum[4];
@@ -544,7 +562,7 @@ __device__ void treelet_rank_tile(const int *__restrict__ tl_of, int n, int *__r
                            __HIP_MEMORY_SCOPE_AGENT);
         unsigned excl = 0;
         if (tile > 0) {
-            excl = look_back(lb, tile, 1, stats);
+            excl = look_back(lb, tile, 1, stats, spin_limit);
             __hip_atomic_store(mine, kFlagInc | (excl + total), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -571,10 +589,11 @@ __global__ __launch_bounds__(256) void k_rank_preorder(
     const int *__restrict__ tl_of, int n, int *__restrict__ trank, int *__restrict__ tsorted,
     unsigned *__restrict__ lb, int *__restrict__ counts, int ntiles,
     unsigned long long *__restrict__ stats, const int *__restrict__ up,
-    const int2 *__restrict__ range, int *__restrict__ pre)
+    const int2 *__restrict__ range, int *__restrict__ pre, int spin_limit)
 {
+    if (tree_aborted(stats)) return;
     if (static_cast<int>(blockIdx.x) < ntiles) {
-        treelet_rank_tile(tl_of, n, trank, tsorted, lb, counts, ntiles, stats);
+        treelet_rank_tile(tl_of, n, trank, tsorted, lb, counts, ntiles, stats, spin_limit);
         return;
     }
     preorder_part(up, range, n, pre, stats, blockIdx.x - ntiles, gridDim.x - ntiles);
@@ -605,6 +624,7 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     const int *__restrict__ tsorted, const int *__restrict__ counts, Agg *agg,
     WalkOut wo)
 {
+    if (tree_aborted(wo.stats)) return;
     constexpr int C = kTreelet;
     __shared__ double s_lx[3][C];   // leaves: position
     __shared__ double s_lm[C];      //         mass
@@ -753,6 +773,15 @@ __global__ __launch_bounds__(256) void k_agg_upper(
     const int *__restrict__ counts, const int *__restrict__ trank,
     const int *__restrict__ tsorted, Agg *agg, WalkOut wo)
 {
+    if (tree_aborted(wo.stats)) {
+        // Every kernel that can give up has finished: tell the host (the
+        // next call on the handle reports it without synchronising).
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            __hip_atomic_store(wo.host_flag, wo.stats[3], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        return;
+    }
     __shared__ double s_r[4][10];
     const int t = threadIdx.x;
     const int w = t >> 6;
@@ -905,6 +934,7 @@ __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restric
                          AccOut out, int n, double G, double theta2, double eps2,
                          unsigned long long *__restrict__ stats)
 {
+    if (tree_aborted(stats)) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
     const bool active = t < n;
@@ -935,7 +965,7 @@ __global__ void k_walk64(const WalkRec64 *__restrict__ rec, const int *__restric
                 node = r.right;
             }
         }
-        if (node >= 0) atomicAdd(&stats[3], 1ull);  // walk did not terminate in bounds
+        if (node >= 0) atomicAdd(&stats[4], 1ull);  // walk did not terminate in bounds
         const int p = order[t];
         store_acc(out, p, ax, ay, az);
     }
@@ -956,6 +986,7 @@ __global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict
                          AccOut out, int n, float eps2,
                          unsigned long long *__restrict__ stats)
 {
+    if (tree_aborted(stats)) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
     const bool active = t < n;
@@ -1010,6 +1041,7 @@ __global__ void k_walk32_pair(const float4 *__restrict__ recA, const int2 *__res
                               AccOut out, int n, float eps2,
                               unsigned long long *__restrict__ stats)
 {
+    if (tree_aborted(stats)) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int g = b * blockDim.x + threadIdx.x;
     const int t = g >> 1;
@@ -1087,6 +1119,7 @@ __global__ __launch_bounds__(256) void k_walk32_wave(const float4 *__restrict__ 
                                                      AccOut out, int n, float eps2,
                                                      unsigned long long *__restrict__ stats)
 {
+    if (tree_aborted(stats)) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -1170,6 +1203,7 @@ __global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__
                                                       float eps2,
                                                       unsigned long long *__restrict__ stats)
 {
+    if (tree_aborted(stats)) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -1283,6 +1317,7 @@ __global__ __launch_bounds__(256) void k_walk32_wave_split(
     const int *__restrict__ order, AccOut out, int n, float eps2,
     unsigned long long *__restrict__ stats)
 {
+    if (tree_aborted(stats)) return;
     constexpr int G = 4 / SPLIT;  // groups of 64 particles per 256-thread block
     __shared__ float s_part[G][SPLIT - 1][64][3];  // waves 1.. partial sums
     __shared__ int s_vis[G][SPLIT - 1][64];
@@ -1438,6 +1473,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     const int *__restrict__ order, AccOut out, int n, float eps2,
     unsigned long long *__restrict__ stats, ChunkShape cs)
 {
+    if (tree_aborted(stats)) return;
     __shared__ int s_b[kChunkMaxB + 1];
     __shared__ int s_nc;
     __shared__ int s_ctr;
@@ -1696,6 +1732,10 @@ struct rg_bh {
     int2 *recB = nullptr;
     double *box = nullptr;    // bbox partials followed by the 4-double box
     unsigned long long *stats = nullptr;
+    // Pinned host word the device sets when a compute aborts (a look-back
+    // wait gave up); read, without synchronising, by the next call.
+    unsigned long long *host_flag = nullptr;
+    unsigned long long *host_flag_dev = nullptr;  // its device address
     std::vector<void *> allocs;
 
     // Optional per-phase timing: 5 events per compute (start, sorted, built,
@@ -1719,6 +1759,8 @@ void bh_free(rg_bh *h)
 {
     for (void *p : h->allocs) (void)hipFree(p);
     h->allocs.clear();
+    if (h->host_flag) (void)hipHostFree(h->host_flag);
+    h->host_flag = nullptr;
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     h->ev.clear();
 }
@@ -1778,8 +1820,9 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     RG_REQUIRE(morton_bits >= 1 && morton_bits <= 21, RG_ERR_ARG, "morton_bits must be in [1, 21]");
     RG_REQUIRE(theta >= 0.0 && std::isfinite(theta), RG_ERR_ARG, "theta must be finite and >= 0");
     RG_REQUIRE(eps > 0.0 && std::isfinite(eps), RG_ERR_ARG, "softening must be > 0");
-    RG_REQUIRE(precision == RG_FP64 || precision == RG_FP32 || precision == RG_FP32C, RG_ERR_ARG,
-               "unknown precision %d", precision);
+    RG_REQUIRE(precision == RG_FP64 || precision == RG_FP32, RG_ERR_ARG,
+               "BH precision must be RG_FP64 or RG_FP32 (the walk has no compensated form), "
+               "got %d", precision);
     rg_bh *h = new (std::nothrow) rg_bh();
     RG_REQUIRE(h, RG_ERR_CAPACITY, "out of host memory");
     h->n = n;
@@ -1838,18 +1881,49 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
         ALLOC(h->recB, 8 * M);
     }
     ALLOC(h->box, 8 * bbox_part_doubles(static_cast<int>(n)));
-    ALLOC(h->stats, 8 * 4);
+    ALLOC(h->stats, 8 * kStatWords);
 #undef ALLOC
+    // Coherent (fine-grained) pinned memory: the device's store reaches the
+    // host without a synchronisation.
+    void *hf = nullptr, *hfd = nullptr;
+    e = hipHostMalloc(&hf, 64, hipHostMallocCoherent);
+    if (e == hipSuccess) {
+        h->host_flag = static_cast<unsigned long long *>(hf);
+        e = hipHostGetDevicePointer(&hfd, hf, 0);
+    }
+    if (e != hipSuccess) {
+        bh_free(h);
+        delete h;
+        return rg::hip_fail(e, "hipHostMalloc", __FILE__, __LINE__);
+    }
+    h->host_flag_dev = static_cast<unsigned long long *>(hfd);
+    __atomic_store_n(h->host_flag, 0ull, __ATOMIC_RELAXED);
     *out = h;
     return RG_OK;
 }
 
 namespace {
 
+// A compute that aborted (a radix-sort or treelet-ranking look-back wait gave
+// up) and has not been reported yet: reports it once and clears the flag.
+int bh_take_abort(rg_bh *h)
+{
+    const unsigned long long f = __atomic_exchange_n(h->host_flag, 0ull, __ATOMIC_ACQ_REL);
+    RG_REQUIRE(f == 0, RG_ERR_CAPACITY,
+               "BH radix sort: %llu look-back waits gave up (sort_spin_limit) in an earlier "
+               "compute on this handle; its tree and accelerations were not computed",
+               f);
+    return RG_OK;
+}
+
 int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &out,
                void *stream, const KickDrift *kd = nullptr)
 {
     RG_REQUIRE(h, RG_ERR_ARG, "null handle");
+    {
+        const int ab = bh_take_abort(h);
+        if (ab != RG_OK) return ab;
+    }
     RG_REQUIRE(pos && mass && out.acc, RG_ERR_ARG, "null pointer argument");
     const int n = static_cast<int>(h->n);
     const size_t b3 = 24 * static_cast<size_t>(n), b1 = 8 * static_cast<size_t>(n);
@@ -1894,7 +1968,7 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
 
     const int g1 = min((n + 255) / 256, 16384);
     k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->up, h->range,
-                                h->left_at_split, h->tl_of, h->rank_lb, h->rank_tiles);
+                                h->left_at_split, h->tl_of, h->rank_lb, h->rank_tiles, h->stats);
     RG_CHECK_LAUNCH();
     const bool fp64 = h->precision == RG_FP64;
     {
@@ -1902,7 +1976,7 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
         const int gp = fp64 ? 0 : min((2 * n - 1 + 255) / 256, 16384);
         k_rank_preorder<<<h->rank_tiles + gp, 256, 0, st>>>(
             h->tl_of, n, h->trank, h->tsorted, h->rank_lb, h->counts, h->rank_tiles, h->stats,
-            h->up, h->range, h->pre);
+            h->up, h->range, h->pre, rg::g_spin_limit);
         RG_CHECK_LAUNCH();
     }
 
@@ -1915,6 +1989,8 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     wo.G = h->G;
     wo.theta_zero = !(theta2 > 0.0);
     wo.inv_theta2 = wo.theta_zero ? 0.0 : 1.0 / theta2;
+    wo.stats = h->stats;
+    wo.host_flag = h->host_flag_dev;
     rc = bh_mark(h, 2, st);
     if (rc != RG_OK) return rc;
     const int gt = min(n / 64 + 64, 4096);
@@ -2089,6 +2165,8 @@ extern "C" int rg_bh_export_tree(rg_bh *h, uint64_t *keys_sorted, int32_t *order
 {
     RG_REQUIRE(h, RG_ERR_ARG, "null handle");
     RG_REQUIRE(h->computed, RG_ERR_ARG, "rg_bh_compute has not run on this handle");
+    RG_REQUIRE(__atomic_load_n(h->host_flag, __ATOMIC_ACQUIRE) == 0, RG_ERR_CAPACITY,
+               "the last compute on this handle aborted (radix-sort look-back gave up)");
     hipStream_t st = rg::as_stream(stream);
     const size_t N = static_cast<size_t>(h->n);
     const hipMemcpyKind k = hipMemcpyDeviceToDevice;
@@ -2105,6 +2183,8 @@ extern "C" int rg_bh_export_nodes(rg_bh *h, double *com, double *nmass, double *
 {
     RG_REQUIRE(h, RG_ERR_ARG, "null handle");
     RG_REQUIRE(h->computed, RG_ERR_ARG, "rg_bh_compute has not run on this handle");
+    RG_REQUIRE(__atomic_load_n(h->host_flag, __ATOMIC_ACQUIRE) == 0, RG_ERR_CAPACITY,
+               "the last compute on this handle aborted (radix-sort look-back gave up)");
     const int n = static_cast<int>(h->n);
     const int g = min((2 * n - 1 + 255) / 256, 16384);
     k_export_nodes<<<g, 256, 0, rg::as_stream(stream)>>>(h->agg, n, com, nmass, nmin, nmax);
@@ -2117,7 +2197,7 @@ extern "C" int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out)
     RG_REQUIRE(h && out, RG_ERR_ARG, "null argument");
     RG_REQUIRE(h->computed, RG_ERR_ARG, "rg_bh_compute has not run on this handle");
     RG_HIP(hipStreamSynchronize(h->last_stream));
-    unsigned long long s[4] = {0, 0, 0, 0};
+    unsigned long long s[5] = {0, 0, 0, 0, 0};
     RG_HIP(hipMemcpy(s, h->stats, sizeof(s), hipMemcpyDeviceToHost));
     out->n = h->n;
     out->visits_total = static_cast<int64_t>(s[0]);
@@ -2125,8 +2205,15 @@ extern "C" int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out)
     out->nonfinite = static_cast<int64_t>(s[2]);
     out->walk_mode = h->walk_used;
     out->walk_waves = h->walk_waves;
-    RG_REQUIRE(s[3] == 0, RG_ERR_CAPACITY,
-               "walk exceeded 2n node visits for %llu particles (corrupt tree)", s[3]);
+    if (s[3] != 0) {
+        __atomic_store_n(h->host_flag, 0ull, __ATOMIC_RELEASE);  // reported here
+        rg::set_error("BH radix sort: %llu look-back waits gave up (sort_spin_limit); the last "
+                      "compute's tree and accelerations were not computed",
+                      s[3]);
+        return RG_ERR_CAPACITY;
+    }
+    RG_REQUIRE(s[4] == 0, RG_ERR_CAPACITY,
+               "walk or preorder pass ran out of bounds for %llu nodes (corrupt tree)", s[4]);
     return RG_OK;
 }
 

@@ -24,6 +24,7 @@ namespace rg {
 
 int g_sort_ticket = 0;
 int g_sort_items = 0;
+int g_spin_limit = kDefaultSpinLimit;
 
 void set_error(const char *fmt, ...)
 {
@@ -82,6 +83,12 @@ extern "C" int rg_set_param(const char *name, int64_t value)
         RG_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16, RG_ERR_ARG,
                    "sort_items must be 0, 4, 8 or 16");
         rg::g_sort_items = static_cast<int>(value);
+        return RG_OK;
+    }
+    if (std::strcmp(name, "sort_spin_limit") == 0) {
+        RG_REQUIRE(value >= -1 && value <= (int64_t(1) << 30), RG_ERR_ARG,
+                   "sort_spin_limit must be -1 (default) or in [0, 2^30]");
+        rg::g_spin_limit = value < 0 ? rg::kDefaultSpinLimit : static_cast<int>(value);
         return RG_OK;
     }
     if (std::strcmp(name, "fp32c_d64") == 0) return rg::rg_direct_set_fp32c_d64_impl(value);

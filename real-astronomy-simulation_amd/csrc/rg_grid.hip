@@ -99,10 +99,12 @@ __global__ __launch_bounds__(kGridBS) void k_grid_setup(const double *__restrict
                                                         double min_radius, long long max_cells,
                                                         GridDev *__restrict__ gd,
                                                         unsigned *__restrict__ ghist,
-                                                        int *__restrict__ tickets)
+                                                        int *__restrict__ tickets,
+                                                        unsigned long long *__restrict__ stats)
 {
     for (int k = threadIdx.x; k < kMaxPasses * kBuckets; k += kGridBS) ghist[k] = 0u;
     if (threadIdx.x < kMaxPasses) tickets[threadIdx.x] = 0;
+    if (threadIdx.x == 0) stats[3] = 0ull;  // this rebuild's look-back give-ups
     __shared__ double s[9][kGridBS];
     const int t = threadIdx.x;
     double v[9] = {0.0, 0.0, INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY,
@@ -349,7 +351,7 @@ extern "C" int rg_grid_rebuild(rg_grid *g, const double *pos, const double *h,
     k_grid_partial<<<g->npart, kGridBS, 0, st>>>(pos, h, is_star, n, g->part);
     RG_CHECK_LAUNCH();
     k_grid_setup<<<1, kGridBS, 0, st>>>(g->part, g->npart, g->min_radius, g->max_cells, g->gd,
-                                        g->ghist, g->tickets);
+                                        g->ghist, g->tickets, g->stats);
     RG_CHECK_LAUNCH();
     const int lb_words = kBuckets * g->nblk * g->passes;
     const int kg = std::min((n + 255) / 256, 256);
@@ -376,7 +378,10 @@ extern "C" int rg_grid_rebuild(rg_grid *g, const double *pos, const double *h,
     unsigned long long flags[4];
     RG_HIP(hipMemcpyAsync(flags, g->stats, sizeof(flags), hipMemcpyDeviceToHost, st));
     RG_HIP(hipStreamSynchronize(st));
-    RG_REQUIRE(flags[3] == 0, RG_ERR_CAPACITY, "sort look-back wait exceeded its bound");
+    RG_REQUIRE(flags[3] == 0, RG_ERR_CAPACITY,
+               "grid radix sort: %llu look-back waits gave up (sort_spin_limit); gsort and "
+               "cell_start are not valid",
+               flags[3]);
     RG_REQUIRE(hd.ngas >= 0, RG_ERR_NONFINITE,
                "%lld gas particles have a non-finite position or smoothing length",
                hd.nonfinite);

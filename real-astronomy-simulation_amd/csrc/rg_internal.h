@@ -33,6 +33,11 @@ extern int g_sort_ticket;
 // (by n), 4, 8 or 16.
 extern int g_sort_items;
 
+// Re-polls a decoupled look-back wait (radix sort, treelet ranking) makes
+// before it gives up and flags the failure; 0 gives up at once.
+extern int g_spin_limit;
+constexpr int kDefaultSpinLimit = 1 << 20;
+
 // Tunables behind rg_set_param.
 int rg_direct_set_ipt_impl(int64_t v);
 int rg_direct_set_split_impl(int64_t v);

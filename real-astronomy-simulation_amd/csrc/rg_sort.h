@@ -54,12 +54,15 @@ constexpr int kMaxPasses = (63 + kDigitBits - 1) / kDigitBits;  // 8 for a 63-bi
 // its exclusive offsets.  A look-back word packs a status (1 = this tile's
 // count, 2 = inclusive prefix) with a 30-bit count, so a single relaxed
 // agent-scope atomic carries both.  Every wait is on a tile that started
-// earlier and publishes without waiting, and is bounded; exceeding the bound
-// flags stats[3] instead of hanging.
+// earlier and publishes without waiting, and is bounded by spin_limit
+// re-polls (rg::g_spin_limit, 2^20 by default; 0 gives up at once).  A wait
+// that gives up adds 1 to stats[3] instead of hanging: the pass's order is
+// then wrong (in bounds, but with repeated and missing entries), so every
+// kernel that consumes it checks stats[3] and does nothing, and the host
+// reports the failure.
 constexpr unsigned kFlagAgg = 1u << 30;
 constexpr unsigned kFlagInc = 2u << 30;
 constexpr unsigned kCountMask = (1u << 30) - 1u;
-constexpr int kSpinLimit = 1 << 20;
 
 // Exclusive prefix over earlier tiles of one look-back word column.  The
 // words of the kLookWin tiles before the current position are loaded
@@ -75,12 +78,17 @@ constexpr int kLookWin = 8;
 
 template <int WIN = kLookWin>
 __device__ __forceinline__ unsigned look_back(const unsigned *__restrict__ lb, int tile,
-                                              int stride, unsigned long long *__restrict__ stats)
+                                              int stride, unsigned long long *__restrict__ stats,
+                                              int spin_limit)
 {
     constexpr int kLookWin = WIN;
     unsigned excl = 0;
     int j = tile - 1;
     int spins = 0;
+    if (spin_limit <= 0) {  // host-forced give-up (tests of the failure path)
+        atomicAdd(&stats[3], 1ull);
+        return 0u;
+    }
     while (j >= 0) {
         unsigned v[kLookWin];
 #pragma unroll
@@ -102,7 +110,7 @@ __device__ __forceinline__ unsigned look_back(const unsigned *__restrict__ lb, i
         if (done) break;
         j -= used;
         if (used < kLookWin) {  // stopped at an unpublished word
-            if (++spins > kSpinLimit) {
+            if (++spins > spin_limit) {
                 atomicAdd(&stats[3], 1ull);
                 break;
             }
@@ -123,7 +131,7 @@ __global__ __launch_bounds__(kSortBS) void k_onesweep(
     const uint64_t *__restrict__ kin, const int *__restrict__ iin, uint64_t *__restrict__ kout,
     int *__restrict__ iout, int n, int shift, const unsigned *__restrict__ ghist_p,
     unsigned *__restrict__ lookback_p, int *__restrict__ ticket_p,
-    unsigned long long *__restrict__ stats)
+    unsigned long long *__restrict__ stats, int spin_limit)
 {
     constexpr int kWaves = kSortBS / 64;
     constexpr int kRounds = ITEMS;             // each wave: kRounds rounds of 64 keys
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(kSortBS) void k_onesweep(
         const int d = t * DPT + q;
         excl[q] = 0;
         if (tile > 0) {
-            excl[q] = look_back(lookback_p + d, tile, kBuckets, stats);
+            excl[q] = look_back(lookback_p + d, tile, kBuckets, stats, spin_limit);
             __hip_atomic_store(&lookback_p[tile * kBuckets + d], kFlagInc | (excl[q] + cnt[q]),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -271,7 +279,7 @@ inline void launch_onesweep(int items, int nblk, hipStream_t st, const uint64_t 
     const bool ticket = rg::g_sort_ticket || nblk > kTicketlessTiles;
 #define RG_ONESWEEP(IT, TK)                                                                   \
     k_onesweep<IT, TK><<<nblk, kSortBS, 0, st>>>(kin, iin, kout, iout, n, shift, ghist_p,     \
-                                                 lookback_p, ticket_p, stats)
+                                                 lookback_p, ticket_p, stats, rg::g_spin_limit)
     if (items == kSortItemsSmall) {
         if (ticket) RG_ONESWEEP(kSortItemsSmall, true);
         else RG_ONESWEEP(kSortItemsSmall, false);

@@ -16,7 +16,14 @@ float64 CUDA tensors and every kernel runs through the HIP C ABI:
 * ``energies()`` is FP64 (KE, PE, KE + PE) like core/engine.py:119-124.
 
 Extra keyword arguments: ``precision`` ("fp64" default = the reference's
-arithmetic, "fp32", "fp32c"), ``G`` (default core.units.G), ``morton_bits``.
+arithmetic, "fp32", "fp32c"; the tree walk has no compensated form, so
+``gravity_mode="bh"`` takes "fp64" or "fp32"), ``G`` (default
+core.units.G), ``morton_bits``.
+
+With ``precision="fp32c"`` the engine holds each position as an FP64 value
+plus an FP32 offset (``pos`` and ``pos_lo``): the drift adds v dt into the
+pair with a compensated sum, the force kernels read both parts, and
+``to_state`` returns their sum rounded to FP64.
 """
 from __future__ import annotations
 
@@ -54,6 +61,9 @@ class Engine:
         self.morton_bits = int(morton_bits)
         self.check_finite = bool(check_finite)
         _lib.precision_code(precision)
+        if gravity_mode == "bh" and precision == "fp32c":
+            raise ValueError("the tree walk has no compensated form: gravity_mode='bh' takes "
+                             "precision 'fp64' or 'fp32'")
         self._bh = None
         self.n = 0
         self.time = 0.0
@@ -98,6 +108,8 @@ class Engine:
         ``Engine.from_state`` resumes with the same precision and tree."""
         meta = dict(self.meta)
         meta.update(self.settings())
+        if self._bh is not None:
+            self._bh.stats()  # raises if a compute since the last check aborted
         return State(pos=self.pos.cpu().numpy(), vel=self.vel.cpu().numpy(),
                      mass=self.mass.cpu().numpy(), ptype=self._ptype, ids=self._ids,
                      time=self.time, step=self.step_count, seed=self.seed, age=self._age,
@@ -163,6 +175,8 @@ class Engine:
         """(kinetic, potential, total), FP64 (core/engine.py:119-124)."""
         ke, pe, _ = gravity.energy_terms(self.pos, self.vel, self.mass, self.n, self.G,
                                          self.softening ** 2)
+        if self._bh is not None:
+            self._bh.stats()  # raises if a compute since the last check aborted
         return ke, pe, ke + pe
 
     def momentum(self):

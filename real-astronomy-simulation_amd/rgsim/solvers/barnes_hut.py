@@ -11,8 +11,13 @@ barnes_hut.py:212).
 
 Extra keyword arguments: ``G`` (the reference uses core.units.G),
 ``precision`` ("fp64": the reference's FP64 criterion and arithmetic;
-"fp32"/"fp32c": FP32 node records), ``morton_bits`` (21 -> 63-bit keys as
-the reference; 10 -> 30-bit keys).
+"fp32": FP32 node records; there is no compensated walk, so "fp32c" is
+rejected), ``morton_bits`` (21 -> 63-bit keys as the reference; 10 ->
+30-bit keys).
+
+Failures are never silent: a compute whose radix-sort look-back wait gave
+up (the ``sort_spin_limit`` bound) computes nothing after the sort, and the
+next call on the handle -- or ``stats()``, which synchronises -- raises.
 """
 from __future__ import annotations
 
@@ -130,7 +135,8 @@ class BarnesHut:
         return com, nmass, nmin, nmax
 
     def stats(self) -> dict:
-        """Walk statistics of the last compute (synchronises its stream)."""
+        """Walk statistics of the last compute (synchronises its stream);
+        raises if that compute aborted or its tree was corrupt."""
         s = _lib.BHStats()
         _lib.check(_lib.load().rg_bh_stats(self._h, ctypes.byref(s)), "rg_bh_stats")
         return {"n": s.n, "visits_total": s.visits_total, "visits_max": s.visits_max,

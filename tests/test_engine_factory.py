@@ -23,7 +23,9 @@ def test_engine_settings_round_trip_through_npz(tmp_path):
     arguments come from them (checked without loading onto a GPU)."""
     from rgsim.engine import Engine
     from rgsim.state import State
-    e = Engine(softening=0.07, gravity_mode="bh", theta=0.45, precision="fp32c",
+    with pytest.raises(ValueError):
+        Engine(gravity_mode="bh", precision="fp32c")  # no compensated tree walk
+    e = Engine(softening=0.07, gravity_mode="bh", theta=0.45, precision="fp32",
                morton_bits=10)
     rng = np.random.default_rng(3)
     s = State(pos=rng.normal(size=(9, 3)), vel=rng.normal(size=(9, 3)), mass=np.ones(9),

@@ -243,24 +243,32 @@ def test_bh_1m_keys_order_topology_bitexact(orc):
     assert np.array_equal(parent, t["parent"])
 
 
-def test_bh_dropin_config_ic_and_after_200_steps(orc):
+@pytest.mark.parametrize("bits", [21, 10])
+def test_bh_dropin_config_ic_and_after_200_steps(orc, bits):
     """The Barnes-Hut drop-in configuration at full size: disk N=100,000 +
-    bulge N=20,000 (seed 1), G 43009.1725, eps 0.05, theta 0.5, 63-bit keys,
-    200 KDK steps at dt 1e-4 through the Engine (engine.py:108-116).  Keys,
-    stable order and topology equal the oracle's on the initial positions
-    and again on the positions after the 200 steps; the force error against
-    the FP64 direct sum stays within verify_bh_galaxy.py:54's 5% mean."""
+    bulge N=20,000 (seed 1), G 43009.1725, eps 0.05, theta 0.5, 200 KDK
+    steps at dt 1e-4 through the Engine (engine.py:108-116), with 63-bit
+    keys (the reference's default, barnes_hut.py:246) and with the 30-bit
+    keys (10 bits per axis, morton_codes' bits argument, barnes_hut.py:38)
+    the configuration names -- where many bulge particles share a key and
+    the index tie-break (barnes_hut.py:78) builds deep chains.  Keys, stable
+    order and topology equal the oracle's on the initial positions and
+    again on the positions after the 200 steps; the force error against the
+    FP64 direct sum stays within verify_bh_galaxy.py:54's 5% mean."""
     from rgsim.engine import Engine
     from rgsim.ic import make_disk_bulge
     from rgsim.solvers import gravity
-    print("bh drop-in config N=120000, 200 steps", flush=True)
+    print(f"bh drop-in config N=120000, {3 * bits}-bit keys, 200 steps", flush=True)
     s = make_disk_bulge(n_disk=100000, n_bulge=20000, seed=1)
-    eng = Engine(softening=0.05, gravity_mode="bh", theta=0.5, precision="fp32")
+    eng = Engine(softening=0.05, gravity_mode="bh", theta=0.5, precision="fp32",
+                 morton_bits=bits)
     eng.load_state(s)
 
     def check_tree(pos_np):
         keys, order, left, right, parent = (t.cpu().numpy() for t in eng._bh.export_tree())
-        t = orc.bh_tree(pos_np, s.mass)
+        t = orc.bh_tree(pos_np, s.mass, bits)
+        if bits == 10:  # the case this parametrization is for: duplicate keys
+            assert len(np.unique(t["keys"])) < len(t["keys"])
         assert np.array_equal(order, t["order"])
         assert np.array_equal(keys.astype(np.uint64), t["keys"][t["order"]])
         assert np.array_equal(left, t["left"]) and np.array_equal(right, t["right"])
@@ -279,3 +287,43 @@ def test_bh_dropin_config_ic_and_after_200_steps(orc):
     print(f"  after 200 steps: force error mean {rel.mean():.2e} max {rel.max():.2e}", flush=True)
     assert rel.mean() < 0.05
     assert eng.step_count == 200
+
+
+def test_sort_lookback_giveup_is_reported(orc):
+    """A radix-sort look-back wait that gives up is never silent.  With
+    sort_spin_limit 0 every wait gives up at once (set from the host, so
+    every kernel still runs to its end); the kernels after the sort then
+    leave acc untouched, the next call on the handle raises with the
+    radix-sort message, and so does stats().  With the default bound
+    restored the same handle computes the oracle's tree again."""
+    from rgsim import _lib
+    from rgsim.solvers.barnes_hut import BarnesHut
+    print("bh sort give-up reporting, N=5000", flush=True)
+    pos, mass = CASES["diskbulge6000"]
+    pos, mass = pos[:5000], mass[:5000]
+    bh = BarnesHut(5000, theta=0.5, softening=0.05, G=1.0, precision="fp32")
+    p, m = dev(pos), dev(mass)
+    acc = torch.full_like(p, 7.0)
+    _lib.set_param("sort_spin_limit", 0)
+    try:
+        bh.compute(p, m, acc)  # asynchronous: returns before the kernels run
+        torch.cuda.synchronize()
+        assert torch.all(acc == 7.0), "an aborted compute wrote acc"
+        with pytest.raises(RuntimeError, match="look-back"):
+            bh.compute(p, m, acc)
+        with pytest.raises(RuntimeError, match="look-back"):
+            bh.stats()
+        # the grid shares the sort: its rebuild synchronises and reports it
+        from rgsim.solvers.neighbor_grid import NeighborGrid
+        g = NeighborGrid(5000, 0.0)
+        with pytest.raises(RuntimeError, match="look-back"):
+            g.rebuild_all(p, torch.full_like(m, 0.1))
+    finally:
+        _lib.set_param("sort_spin_limit", -1)
+    bh.compute(p, m, acc)
+    bh.stats()
+    keys, order, left, right, parent = (t.cpu().numpy() for t in bh.export_tree())
+    t = orc.bh_tree(pos, mass)
+    assert np.array_equal(order, t["order"]) and np.array_equal(parent, t["parent"])
+    assert torch.all(torch.isfinite(acc))
+    g.rebuild_all(p, torch.full_like(m, 0.1))
