@@ -112,6 +112,21 @@ int rg_set_param(const char *name, int64_t value);
 int rg_direct_acc(const double *pos, const double *mass, double *acc, int64_t n,
                   double G, double eps2, int precision, void *stream);
 
+/* rg_direct_acc with no allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels,
+ * launch shapes and results (bit for bit) as rg_direct_acc; the FP32/FP32C
+ * j-slice partials go to a static 128 MiB device array that each GPU holds
+ * from the library's first launch on it until the process ends, instead of
+ * the library's per-stream buffer.  n whose split needs more (3 doubles x
+ * slices x n; with the default split, n up to 87,381 in every precision)
+ * is rejected with RG_ERR_CAPACITY before any launch.  The array needs no
+ * clearing (every slice overwrites its partials), but every call on one GPU
+ * shares it: calls that overlap on one GPU (other streams, other host
+ * threads) race. */
+int rg_direct_acc_noalloc(const double *pos, const double *mass, double *acc, int64_t n,
+                          double G, double eps2, int precision, void *stream);
+
 /* rg_direct_acc followed by leapfrog.kick(vel, acc, n, dt_half)
  * (core/integrators/leapfrog.py:11-15) fused into the force kernel's
  * epilogue: the closing half-kick of Engine.step (core/engine.py:113-114). */

@@ -18,10 +18,26 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <mutex>
 
 #include "rg_internal.h"
 
+// Scratch of rg_direct_acc_noalloc: the FP32/FP32C slice partials live in
+// this static array instead of library-allocated memory.  Each GPU holds its
+// own copy from the library's first launch on it until the process ends;
+// host code uses the per-device address from hipGetSymbolAddress, never the
+// array's name (which is a host-side stand-in).
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
+
+// Where a launch's scratch comes from: the library's grow-only buffer for
+// the stream (ptr == nullptr), or a fixed region of cap bytes.
+struct ScratchSrc {
+    void *ptr;
+    size_t cap;
+};
 
 constexpr int kBS = 256;      // threads per block = j-tile length
 constexpr int kChunk = 32;    // FP32C: pair terms per partial before the two-sum fold
@@ -470,7 +486,8 @@ __global__ __launch_bounds__(256) void k_direct_reduce(const double *__restrict_
 
 template <int MODE>
 int launch_direct(const double *pos, const double *mass, double *acc, double *vel, int n,
-                  double G, double eps2, double dt_half, const Epilogue &epi, hipStream_t st)
+                  double G, double eps2, double dt_half, const Epilogue &epi, hipStream_t st,
+                  const ScratchSrc &ss)
 {
     int ipt = g_direct_ipt;
     if (ipt == 0) {
@@ -484,11 +501,19 @@ int launch_direct(const double *pos, const double *mass, double *acc, double *ve
     const int ns = MODE == 0 ? 1 : rg::direct_slices(n, ipt, &jlen);
     double *part = nullptr;
     if (ns > 1) {
-        void *scratch = nullptr;
-        const int rc = rg::device_scratch(sizeof(double) * 3 * static_cast<size_t>(ns) * n,
-                                          st, &scratch);
-        if (rc != RG_OK) return rc;
-        part = static_cast<double *>(scratch);
+        const size_t need = sizeof(double) * 3 * static_cast<size_t>(ns) * n;
+        void *buf = nullptr;
+        if (ss.ptr) {
+            RG_REQUIRE(need <= ss.cap, RG_ERR_CAPACITY,
+                       "n=%d with %d j-slices needs %zu MiB of slice partials; the static "
+                       "scratch holds %zu MiB (lower \"direct_split\" or use rg_direct_acc)",
+                       n, ns, (need + (1u << 20) - 1) >> 20, ss.cap >> 20);
+            buf = ss.ptr;
+        } else {
+            const int rc = rg::device_scratch(need, st, &buf);
+            if (rc != RG_OK) return rc;
+        }
+        part = static_cast<double *>(buf);
     }
     const dim3 grid(ib, ns);
     switch (ipt) {
@@ -579,7 +604,7 @@ int launch_direct64_pipe(int w, const double *pos, const double *mass, double *a
 
 int direct_common(const double *pos, const double *mass, double *acc, double *vel,
                   int64_t n, double G, double eps2, int precision, double dt_half,
-                  const Epilogue &epi, void *stream)
+                  const Epilogue &epi, void *stream, const ScratchSrc &ss = {nullptr, 0})
 {
     RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY,
                "n=%lld outside [0, 2^29]", static_cast<long long>(n));
@@ -600,13 +625,13 @@ int direct_common(const double *pos, const double *mass, double *acc, double *ve
     case RG_FP64: {
         const int w = fp64_pipe_waves(ni);
         if (w > 1) return launch_direct64_pipe(w, pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
-        return launch_direct<0>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
+        return launch_direct<0>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
     }
-    case RG_FP32: return launch_direct<1>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
+    case RG_FP32: return launch_direct<1>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
     case RG_FP32C:
         if (g_fp32c_d64)
-            return launch_direct<3>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
-        return launch_direct<2>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
+            return launch_direct<3>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
+        return launch_direct<2>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
     default: break;
     }
     rg::set_error("unknown precision %d", precision);
@@ -795,6 +820,44 @@ extern "C" int rg_direct_acc(const double *pos, const double *mass, double *acc,
                          make_epilogue(kStore, nullptr), stream);
 }
 
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices];  // device address of `scratch`, per device
+
+// The current device's copy of `scratch`, looked up once per device with
+// that device current.
+int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    RG_HIP(hipGetDevice(&dev));
+    RG_REQUIRE(dev >= 0 && dev < kMaxDevices, RG_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        RG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        RG_REQUIRE(p, RG_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return RG_OK;
+}
+
+}  // namespace
+
+extern "C" int rg_direct_acc_noalloc(const double *pos, const double *mass, double *acc,
+                                     int64_t n, double G, double eps2, int precision,
+                                     void *stream)
+{
+    void *p = nullptr;
+    const int rc = noalloc_scratch(&p);
+    if (rc != RG_OK) return rc;
+    return direct_common(pos, mass, acc, nullptr, n, G, eps2, precision, 0.0,
+                         make_epilogue(kStore, nullptr), stream,
+                         ScratchSrc{p, kNoAllocScratchBytes});
+}
+
 extern "C" int rg_direct_acc_add(const double *pos, const double *mass, double *acc, int64_t n,
                                  double G, double eps2, int precision, void *stream)
 {
@@ -901,12 +964,12 @@ extern "C" int rg_energy(const double *pos, const double *vel, const double *mas
     // the sequential sum)
     int jlen = ni;
     const int ns = pos ? rg::direct_slices(ni, 1, &jlen) : 1;
-    void *scratch = nullptr;
+    void *buf = nullptr;
     hipStream_t s = rg::as_stream(stream);
     const int st = rg::device_scratch(sizeof(double) * 5 * static_cast<size_t>(nb) * ns, s,
-                                      &scratch);
+                                      &buf);
     if (st != RG_OK) return st;
-    double *part = static_cast<double *>(scratch);
+    double *part = static_cast<double *>(buf);
     k_energy_partial<<<dim3(nb, ns), kEB, 0, s>>>(pos, vel, mass, ni, G, eps2, jlen, part);
     RG_CHECK_LAUNCH();
     k_energy_final<<<1, kEB, 0, s>>>(part, nb * ns, out5);

@@ -55,6 +55,7 @@ SIGNATURES = {
     "rg_last_error": (ctypes.c_char_p, []),
     "rg_set_param": (_i, [ctypes.c_char_p, _i64]),
     "rg_direct_acc": (_i, [_vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
+    "rg_direct_acc_noalloc": (_i, [_vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
     "rg_direct_acc_kick": (_i, [_vp, _vp, _vp, _vp, _i64, _d, _d, _i, _d, _vp]),
     "rg_direct_acc_add": (_i, [_vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
     "rg_direct_ext_acc_add": (_i, [_vp, _vp, _vp, _i64, _d, _d, ctypes.POINTER(Potential), _i,
