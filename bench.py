@@ -589,8 +589,10 @@ def bench_bh_1m(steps=5):
             gbs = rec["hbm_bytes_per_launch"] / (walk_ms * 1e-3) / 1e9
             out["walk_traffic"] = {"bytes_per_launch": rec["hbm_bytes_per_launch"],
                                    "GB_per_s": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
-    log(f"bh 1M: {out['ms_per_step']:.2f} ms/step, walk {walk_ms:.2f} ms "
-        f"({out['walk_frac_of_hbm_peak']:.2f} of HBM peak)")
+    frac = out.get("walk_traffic", {}).get("frac_of_hbm_peak")
+    log(f"bh 1M: {out['ms_per_step']:.2f} ms/step, walk {walk_ms:.2f} ms"
+        + (f" (measured traffic {frac:.2f} of HBM peak)" if frac is not None else "")
+        + f", force error mean {err['mean']:.2e} against FP64 direct")
     return out
 
 
