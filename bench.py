@@ -48,6 +48,7 @@ WALK_KERNELS = {0: "k_walk32", 1: "k_walk32_pair", 2: "k_walk32_wave", 3: "k_wal
                 4: "k_walk32_wave_split<2>", 5: "k_walk32_wave_split<4>",
                 6: "k_walk32_chunked<{w}>", 7: "k_walk32_chunked<{w}, true>", -1: "k_walk64"}
 WALK_PARTICLE_BYTES = 16 + 4 + 24  # own leaf record + order index + FP64 acceleration
+HILO_KD_BYTES = (24 + 12 + 24 + 24) + (24 + 12 + 24)  # hi/lo kick + drift, read + written
 WALK_NODE_BYTES = 16 + 8           # FP32 record pair (com, G m) + (threshold, skip), per node
 WALK_IO_BYTES = 4 + 24             # order index read + FP64 acceleration written, per particle
 
@@ -387,10 +388,46 @@ def bench_direct(n=262144, evals=3, kdk_steps=50):
     ke1, pe1, e_1 = eng.energies()
     p1 = np.array(eng.momentum())
     out["fp32c_kdk"] = {"steps": kdk_steps, "dt": 5e-4, "ms_per_step": 1e3 * wall / kdk_steps,
+                        "positions": "FP64 hi + FP32 lo (compensated drift)",
                         "energy_drift": abs((e_1 - e_0) / e_0), "virial": -2 * ke1 / pe1,
                         "momentum_change": float(np.linalg.norm(p1 - p0))}
     log(f"direct fp32c KDK: {out['fp32c_kdk']['ms_per_step']:.2f} ms/step, "
         f"|dE/E| {out['fp32c_kdk']['energy_drift']:.2e}")
+    out["hilo_drift"] = bench_hilo_drift(eng)
+    return out
+
+
+def bench_hilo_drift(eng, reps=50):
+    """The opening kick + compensated drift of hi/lo positions
+    (k_kick_drift_hilo): reads x_hi 24 + x_lo 12 + v 24 + a 24 B and writes
+    x_hi 24 + x_lo 12 + v 24 B per particle, 144 B; at the precision form's
+    n and at 2^22 particles, where the pass is long enough to stream at HBM
+    rate.  dt = 0 leaves the positions unchanged."""
+    import torch
+
+    from rgsim.integrators import leapfrog
+
+    def timed(pos, lo, vel, acc, n):
+        leapfrog.kick_drift_hilo(pos, lo, vel, acc, n, 0.0, 0.0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            leapfrog.kick_drift_hilo(pos, lo, vel, acc, n, 0.0, 0.0)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = HILO_KD_BYTES * n / (ms * 1e-3) / 1e9
+        return {"n": n, "ms": ms, "GB_per_s": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+
+    out = {"bytes_per_particle": HILO_KD_BYTES,
+           "at_precision_form_n": timed(eng.pos, eng.pos_lo, eng.vel, eng.acc, eng.n)}
+    big = 1 << 22
+    pos = torch.zeros((big, 3), dtype=torch.float64, device="cuda")
+    lo = torch.zeros((big, 3), dtype=torch.float32, device="cuda")
+    vel, acc = torch.zeros_like(pos), torch.zeros_like(pos)
+    out["at_4M"] = timed(pos, lo, vel, acc, big)
+    log(f"hi/lo kick+drift: {out['at_4M']['GB_per_s']:.0f} GB/s at 2^22 "
+        f"({out['at_4M']['frac_of_hbm_peak']:.2f} of HBM peak)")
     return out
 
 

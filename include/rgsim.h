@@ -134,6 +134,36 @@ int rg_direct_acc_kick(const double *pos, const double *mass, double *acc,
                        double *vel, int64_t n, double G, double eps2,
                        int precision, double dt_half, void *stream);
 
+/* ---------------------------------------- hi/lo positions (precision form)
+ * No reference counterpart: the reference holds FP64 positions only.  A
+ * position is X = pos + pos_lo, pos (n,3) FP64 and pos_lo (n,3) FP32 an
+ * offset of at most half an ulp of pos (zero for positions that are plain
+ * doubles), so pos is X rounded to FP64.  The drift below keeps the sub-ulp
+ * part of every increment that an FP64 drift rounds away. */
+
+/* rg_direct_acc with FP32C pair arithmetic on hi/lo positions: each
+ * position is split into FP32 hi + lo parts from both arrays
+ * (gravity.py:13-26 otherwise, in the FP32C form). */
+int rg_direct_acc_hilo(const double *pos, const float *pos_lo, const double *mass,
+                       double *acc, int64_t n, double G, double eps2, void *stream);
+
+/* rg_direct_acc_hilo followed by leapfrog.kick(vel, acc, n, dt_half)
+ * (core/integrators/leapfrog.py:11-15) in the force kernel's epilogue. */
+int rg_direct_acc_kick_hilo(const double *pos, const float *pos_lo, const double *mass,
+                            double *acc, double *vel, int64_t n, double G, double eps2,
+                            double dt_half, void *stream);
+
+/* leapfrog.drift (core/integrators/leapfrog.py:18-22) on hi/lo positions,
+ * compensated: per component t = pos_lo + vel * dt, (pos, e) = TwoSum(pos,
+ * t), pos_lo = e rounded to FP32. */
+int rg_drift_hilo(double *pos, float *pos_lo, const double *vel, int64_t n, double dt,
+                  void *stream);
+
+/* leapfrog.kick(dt_half) then rg_drift_hilo(dt) in one pass (the opening
+ * half of Engine.step, core/engine.py:111-112). */
+int rg_kick_drift_hilo(double *pos, float *pos_lo, double *vel, const double *acc, int64_t n,
+                       double dt_half, double dt, void *stream);
+
 /* ------------------------------------- self-gravity of the other engines */
 
 /* The analytic galaxy potential the gas-disk and living-galaxy engines add

@@ -90,6 +90,33 @@ __device__ __forceinline__ void split_df(double x, float &hi, float &lo)
     lo = static_cast<float>(x - static_cast<double>(hi));
 }
 
+// The same split of a hi/lo position X = x + off (x FP64, off an FP32
+// offset): hi = x rounded to FP32, lo = ((x - hi) + off) rounded to FP32;
+// x - hi is exact, so hi + lo carries X to about 2^-48 of itself.
+__device__ __forceinline__ void split_df_off(double x, float off, float &hi, float &lo)
+{
+    hi = static_cast<float>(x);
+    lo = static_cast<float>((x - static_cast<double>(hi)) + static_cast<double>(off));
+}
+
+// Compensated drift of a hi/lo coordinate X = xh + xl (xh FP64, xl an FP32
+// offset of at most half an ulp of xh): X += v dt.  The increment (v * dt,
+// rounded once, as leapfrog.py:21 rounds it) is added to the offset first,
+// then TwoSum(xh, t) renormalises: xh takes the rounded sum and xl its
+// exact rounding error, rounded to FP32.  A step therefore keeps the
+// increment to about 2^-53 of itself, where an FP64 drift rounds the new
+// position to half an ulp of |X|; the FP32 rounding of the offset loses at
+// most 2^-25 ulp(xh) per step.
+__device__ __forceinline__ void drift_hilo(double &xh, float &xl, double v, double dt)
+{
+    const double t = static_cast<double>(xl) + v * dt;
+    const double s = xh + t;
+    const double bb = s - xh;
+    const double e = (xh - (s - bb)) + (t - bb);
+    xh = s;
+    xl = static_cast<float>(e);
+}
+
 // What the kernel does with an i-particle's sum once the j loop is done.
 enum EpiMode { kStore = 0, kAdd = 1, kAddExt = 2 };
 
@@ -200,8 +227,13 @@ __device__ __forceinline__ void finish_particle(int i, double ax, double ay, dou
 // writes its slice's sum as FP64 to part[(y*3 + axis)*n + i] and
 // k_direct_reduce adds the slices in order.  FP64 always runs one slice, so
 // its sum keeps the sequential j order of gravity.py:21.
-template <int MODE, int IPT>
+//
+// HILO (MODE 2 only): pos_lo holds the FP32 offsets of hi/lo positions, X =
+// pos + pos_lo, split by split_df_off.  A template flag, so that the plain
+// kernels compile as they do without it.
+template <int MODE, int IPT, bool HILO = false>
 __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
+                                                const float *__restrict__ pos_lo,
                                                 const double *__restrict__ mass,
                                                 double *__restrict__ acc,
                                                 double *__restrict__ vel, int n, double G,
@@ -231,9 +263,15 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
         const double pz = ok ? pos[3 * i + 2] : 0.0;
         xd[k] = px; yd[k] = py; zd[k] = pz;
         axd[k] = ayd[k] = azd[k] = 0.0;
-        split_df(px, xh[k], xl[k]);
-        split_df(py, yh[k], yl[k]);
-        split_df(pz, zh[k], zl[k]);
+        if constexpr (HILO) {
+            split_df_off(px, ok ? pos_lo[3 * i] : 0.f, xh[k], xl[k]);
+            split_df_off(py, ok ? pos_lo[3 * i + 1] : 0.f, yh[k], yl[k]);
+            split_df_off(pz, ok ? pos_lo[3 * i + 2] : 0.f, zh[k], zl[k]);
+        } else {
+            split_df(px, xh[k], xl[k]);
+            split_df(py, yh[k], yl[k]);
+            split_df(pz, zh[k], zl[k]);
+        }
         sx[k] = sy[k] = sz[k] = 0.f;
         cx[k] = cy[k] = cz[k] = 0.f;
     }
@@ -255,9 +293,15 @@ __global__ __launch_bounds__(kBS) void k_direct(const double *__restrict__ pos,
                                         static_cast<float>(qz), static_cast<float>(G * qm));
             } else {
                 float hx, lx, hy, ly, hz, lz;
-                split_df(qx, hx, lx);
-                split_df(qy, hy, ly);
-                split_df(qz, hz, lz);
+                if constexpr (HILO) {
+                    split_df_off(qx, pos_lo[3 * j], hx, lx);
+                    split_df_off(qy, pos_lo[3 * j + 1], hy, ly);
+                    split_df_off(qz, pos_lo[3 * j + 2], hz, lz);
+                } else {
+                    split_df(qx, hx, lx);
+                    split_df(qy, hy, ly);
+                    split_df(qz, hz, lz);
+                }
                 s_hi[tid] = make_float4(hx, hy, hz, static_cast<float>(G * qm));
                 s_lo[tid] = make_float4(lx, ly, lz, 0.f);
             }
@@ -485,9 +529,9 @@ __global__ __launch_bounds__(256) void k_direct_reduce(const double *__restrict_
 }
 
 template <int MODE>
-int launch_direct(const double *pos, const double *mass, double *acc, double *vel, int n,
-                  double G, double eps2, double dt_half, const Epilogue &epi, hipStream_t st,
-                  const ScratchSrc &ss)
+int launch_direct(const double *pos, const float *pos_lo, const double *mass, double *acc,
+                  double *vel, int n, double G, double eps2, double dt_half,
+                  const Epilogue &epi, hipStream_t st, const ScratchSrc &ss)
 {
     int ipt = g_direct_ipt;
     if (ipt == 0) {
@@ -516,23 +560,32 @@ int launch_direct(const double *pos, const double *mass, double *acc, double *ve
         part = static_cast<double *>(buf);
     }
     const dim3 grid(ib, ns);
+    constexpr bool kCanHilo = MODE == 2;
+#define RG_DIRECT_LAUNCH(IPT)                                                                  \
+    do {                                                                                       \
+        if (kCanHilo && pos_lo)                                                                \
+            k_direct<MODE, IPT, kCanHilo><<<grid, kBS, 0, st>>>(pos, pos_lo, mass, acc, vel, n, \
+                                                                G, eps2, dt_half, epi, jlen,   \
+                                                                part);                         \
+        else                                                                                   \
+            k_direct<MODE, IPT><<<grid, kBS, 0, st>>>(pos, nullptr, mass, acc, vel, n, G, eps2, \
+                                                      dt_half, epi, jlen, part);               \
+    } while (0)
     switch (ipt) {
     case 1:
-        k_direct<MODE, 1><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
-                                                  epi, jlen, part);
+        RG_DIRECT_LAUNCH(1);
         break;
     case 2:
-        k_direct<MODE, 2><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
-                                                  epi, jlen, part);
+        RG_DIRECT_LAUNCH(2);
         break;
     case 4:
-        k_direct<MODE, 4><<<grid, kBS, 0, st>>>(pos, mass, acc, vel, n, G, eps2, dt_half,
-                                                  epi, jlen, part);
+        RG_DIRECT_LAUNCH(4);
         break;
     default:
         rg::set_error("direct_ipt must be 1, 2 or 4 (got %d)", ipt);
         return RG_ERR_ARG;
     }
+#undef RG_DIRECT_LAUNCH
     RG_CHECK_LAUNCH();
     if (part) {
         k_direct_reduce<<<elementwise_grid(n), 256, 0, st>>>(part, ns, n, pos, acc, vel, G,
@@ -604,7 +657,8 @@ int launch_direct64_pipe(int w, const double *pos, const double *mass, double *a
 
 int direct_common(const double *pos, const double *mass, double *acc, double *vel,
                   int64_t n, double G, double eps2, int precision, double dt_half,
-                  const Epilogue &epi, void *stream, const ScratchSrc &ss = {nullptr, 0})
+                  const Epilogue &epi, void *stream, const ScratchSrc &ss = {nullptr, 0},
+                  const float *pos_lo = nullptr)
 {
     RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY,
                "n=%lld outside [0, 2^29]", static_cast<long long>(n));
@@ -619,19 +673,28 @@ int direct_common(const double *pos, const double *mass, double *acc, double *ve
         RG_REQUIRE(!rg_overlap(vel, b3, pos, b3) && !rg_overlap(vel, b3, mass, b1) &&
                        !rg_overlap(vel, b3, acc, b3),
                    RG_ERR_ALIAS, "vel aliases pos, mass or acc");
+    if (pos_lo) {
+        RG_REQUIRE(precision == RG_FP32C && !g_fp32c_d64, RG_ERR_ARG,
+                   "hi/lo positions take the FP32C pair sum (with fp32c_d64 off)");
+        const size_t bl = sizeof(float) * 3 * static_cast<size_t>(n);
+        RG_REQUIRE(!rg_overlap(acc, b3, pos_lo, bl) && (!vel || !rg_overlap(vel, b3, pos_lo, bl)),
+                   RG_ERR_ALIAS, "acc or vel aliases pos_lo");
+    }
     hipStream_t st = rg::as_stream(stream);
     const int ni = static_cast<int>(n);
     switch (precision) {
     case RG_FP64: {
         const int w = fp64_pipe_waves(ni);
         if (w > 1) return launch_direct64_pipe(w, pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st);
-        return launch_direct<0>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
+        return launch_direct<0>(pos, nullptr, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
     }
-    case RG_FP32: return launch_direct<1>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
+    case RG_FP32:
+        return launch_direct<1>(pos, nullptr, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
     case RG_FP32C:
         if (g_fp32c_d64)
-            return launch_direct<3>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
-        return launch_direct<2>(pos, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
+            return launch_direct<3>(pos, nullptr, mass, acc, vel, ni, G, eps2, dt_half, epi, st,
+                                    ss);
+        return launch_direct<2>(pos, pos_lo, mass, acc, vel, ni, G, eps2, dt_half, epi, st, ss);
     default: break;
     }
     rg::set_error("unknown precision %d", precision);
@@ -658,6 +721,59 @@ __global__ void k_kick_drift(double *__restrict__ x, double *__restrict__ v,
         const double vk = v[k] + a[k] * h;
         v[k] = vk;
         x[k] = x[k] + vk * dt;
+    }
+}
+
+// leapfrog.drift (leapfrog.py:18-22) on hi/lo positions: X = x + xl += v dt
+// per component, compensated (drift_hilo).
+__global__ void k_drift_hilo(double *__restrict__ x, float *__restrict__ xl,
+                             const double *__restrict__ v, int m, double dt)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        double xh = x[k];
+        float lo = xl[k];
+        drift_hilo(xh, lo, v[k], dt);
+        x[k] = xh;
+        xl[k] = lo;
+    }
+}
+
+// kick (v += a h, leapfrog.py:14) then the hi/lo drift with the new v.
+__global__ void k_kick_drift_hilo(double *__restrict__ x, float *__restrict__ xl,
+                                  double *__restrict__ v, const double *__restrict__ a, int m,
+                                  double h, double dt)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        const double vk = v[k] + a[k] * h;
+        v[k] = vk;
+        double xh = x[k];
+        float lo = xl[k];
+        drift_hilo(xh, lo, vk, dt);
+        x[k] = xh;
+        xl[k] = lo;
+    }
+}
+
+// k_kick_drift_hilo two components per thread (16-byte FP64 and 8-byte FP32
+// accesses): pairs = m / 2 pairs of components; an odd last component is
+// done by k_kick_drift_hilo on its own.
+__global__ void k_kick_drift_hilo2(double2 *__restrict__ x, float2 *__restrict__ xl,
+                                   double2 *__restrict__ v, const double2 *__restrict__ a,
+                                   int pairs, double h, double dt)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < pairs;
+         k += gridDim.x * blockDim.x) {
+        double2 vk = v[k];
+        const double2 ak = a[k];
+        double2 xk = x[k];
+        float2 lk = xl[k];
+        vk.x = vk.x + ak.x * h;
+        vk.y = vk.y + ak.y * h;
+        v[k] = vk;
+        drift_hilo(xk.x, lk.x, vk.x, dt);
+        drift_hilo(xk.y, lk.y, vk.y, dt);
+        x[k] = xk;
+        xl[k] = lk;
     }
 }
 
@@ -909,6 +1025,85 @@ extern "C" int rg_direct_acc_kick(const double *pos, const double *mass, double 
     RG_REQUIRE(vel || n == 0, RG_ERR_ARG, "null vel");
     return direct_common(pos, mass, acc, vel, n, G, eps2, precision, dt_half,
                          make_epilogue(kStore, nullptr), stream);
+}
+
+extern "C" int rg_direct_acc_hilo(const double *pos, const float *pos_lo, const double *mass,
+                                  double *acc, int64_t n, double G, double eps2, void *stream)
+{
+    RG_REQUIRE(n == 0 || pos_lo, RG_ERR_ARG, "null pos_lo");
+    return direct_common(pos, mass, acc, nullptr, n, G, eps2, RG_FP32C, 0.0,
+                         make_epilogue(kStore, nullptr), stream, ScratchSrc{nullptr, 0}, pos_lo);
+}
+
+extern "C" int rg_direct_acc_kick_hilo(const double *pos, const float *pos_lo,
+                                       const double *mass, double *acc, double *vel, int64_t n,
+                                       double G, double eps2, double dt_half, void *stream)
+{
+    RG_REQUIRE(n == 0 || (vel && pos_lo), RG_ERR_ARG, "null vel or pos_lo");
+    return direct_common(pos, mass, acc, vel, n, G, eps2, RG_FP32C, dt_half,
+                         make_epilogue(kStore, nullptr), stream, ScratchSrc{nullptr, 0}, pos_lo);
+}
+
+namespace {
+
+int hilo_check(const double *pos, const float *pos_lo, const void *other, int64_t n)
+{
+    RG_REQUIRE(n >= 0 && n <= (int64_t(1) << 29), RG_ERR_CAPACITY, "n out of range");
+    if (n == 0) return RG_OK;
+    RG_REQUIRE(pos && pos_lo && other, RG_ERR_ARG, "null pointer argument");
+    const size_t b3 = 24 * static_cast<size_t>(n), bl = 12 * static_cast<size_t>(n);
+    RG_REQUIRE(!rg_overlap(pos, b3, pos_lo, bl) && !rg_overlap(pos, b3, other, b3) &&
+                   !rg_overlap(pos_lo, bl, other, b3),
+               RG_ERR_ALIAS, "pos, pos_lo and vel must not overlap");
+    return RG_OK;
+}
+
+}  // namespace
+
+extern "C" int rg_drift_hilo(double *pos, float *pos_lo, const double *vel, int64_t n, double dt,
+                             void *stream)
+{
+    const int rc = hilo_check(pos, pos_lo, vel, n);
+    if (rc != RG_OK || n == 0) return rc;
+    const int m = static_cast<int>(3 * n);
+    k_drift_hilo<<<elementwise_grid(m), 256, 0, rg::as_stream(stream)>>>(pos, pos_lo, vel, m, dt);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
+extern "C" int rg_kick_drift_hilo(double *pos, float *pos_lo, double *vel, const double *acc,
+                                  int64_t n, double dt_half, double dt, void *stream)
+{
+    int rc = hilo_check(pos, pos_lo, vel, n);
+    if (rc != RG_OK || n == 0) return rc;
+    RG_REQUIRE(acc, RG_ERR_ARG, "null acc");
+    const size_t b3 = 24 * static_cast<size_t>(n), bl = 12 * static_cast<size_t>(n);
+    RG_REQUIRE(!rg_overlap(acc, b3, vel, b3) && !rg_overlap(acc, b3, pos, b3) &&
+                   !rg_overlap(acc, b3, pos_lo, bl),
+               RG_ERR_ALIAS, "acc aliases vel, pos or pos_lo");
+    const int m = static_cast<int>(3 * n);
+    hipStream_t st = rg::as_stream(stream);
+    const auto aligned = [](const void *p, uintptr_t b) {
+        return (reinterpret_cast<uintptr_t>(p) & (b - 1)) == 0;
+    };
+    if (aligned(pos, 16) && aligned(vel, 16) && aligned(acc, 16) && aligned(pos_lo, 8)) {
+        const int pairs = m / 2;
+        k_kick_drift_hilo2<<<elementwise_grid(pairs), 256, 0, st>>>(
+            reinterpret_cast<double2 *>(pos), reinterpret_cast<float2 *>(pos_lo),
+            reinterpret_cast<double2 *>(vel), reinterpret_cast<const double2 *>(acc), pairs,
+            dt_half, dt);
+        RG_CHECK_LAUNCH();
+        if (m & 1) {
+            const int last = m - 1;
+            k_kick_drift_hilo<<<1, 64, 0, st>>>(pos + last, pos_lo + last, vel + last,
+                                                acc + last, 1, dt_half, dt);
+            RG_CHECK_LAUNCH();
+        }
+        return RG_OK;
+    }
+    k_kick_drift_hilo<<<elementwise_grid(m), 256, 0, st>>>(pos, pos_lo, vel, acc, m, dt_half, dt);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
 }
 
 extern "C" int rg_kick(double *vel, const double *acc, int64_t n, double dt_half, void *stream)

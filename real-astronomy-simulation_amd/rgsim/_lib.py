@@ -57,6 +57,10 @@ SIGNATURES = {
     "rg_direct_acc": (_i, [_vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
     "rg_direct_acc_noalloc": (_i, [_vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
     "rg_direct_acc_kick": (_i, [_vp, _vp, _vp, _vp, _i64, _d, _d, _i, _d, _vp]),
+    "rg_direct_acc_hilo": (_i, [_vp, _vp, _vp, _vp, _i64, _d, _d, _vp]),
+    "rg_direct_acc_kick_hilo": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _d, _d, _d, _vp]),
+    "rg_drift_hilo": (_i, [_vp, _vp, _vp, _i64, _d, _vp]),
+    "rg_kick_drift_hilo": (_i, [_vp, _vp, _vp, _vp, _i64, _d, _d, _vp]),
     "rg_direct_acc_add": (_i, [_vp, _vp, _vp, _i64, _d, _d, _i, _vp]),
     "rg_direct_ext_acc_add": (_i, [_vp, _vp, _vp, _i64, _d, _d, ctypes.POINTER(Potential), _i,
                                    _vp]),

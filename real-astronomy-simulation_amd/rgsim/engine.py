@@ -71,6 +71,7 @@ class Engine:
         self.seed = 0
         self.meta: dict = {}
         self.pos = self.vel = self.acc = self.mass = None
+        self.pos_lo = None  # FP32 position offsets (precision "fp32c")
         self._ptype = self._ids = None
         self._age = self._temperature = self._metallicity = None
 
@@ -93,6 +94,11 @@ class Engine:
         self.vel = torch.from_numpy(np.ascontiguousarray(state.vel, np.float64)).to(dev)
         self.mass = torch.from_numpy(np.ascontiguousarray(state.mass, np.float64)).to(dev)
         self.acc = torch.empty_like(self.pos)
+        self.pos_lo = None
+        if self.precision == "fp32c":
+            lo = state.pos_lo if state.pos_lo is not None else np.zeros(state.pos.shape,
+                                                                        np.float32)
+            self.pos_lo = torch.from_numpy(np.ascontiguousarray(lo, np.float32)).to(dev)
         if self.gravity_mode == "bh" and self.n >= BH_MIN_N:
             if self._bh is None or self._bh.n != self.n:
                 self._bh = BarnesHut(self.n, theta=self.theta, softening=self.softening,
@@ -114,7 +120,8 @@ class Engine:
                      mass=self.mass.cpu().numpy(), ptype=self._ptype, ids=self._ids,
                      time=self.time, step=self.step_count, seed=self.seed, age=self._age,
                      temperature=self._temperature, metallicity=self._metallicity,
-                     meta=meta)
+                     meta=meta,
+                     pos_lo=self.pos_lo.cpu().numpy() if self.pos_lo is not None else None)
 
     def settings(self) -> dict:
         """The constructor arguments, as snapshot meta entries."""
@@ -145,6 +152,9 @@ class Engine:
             self._bh.compute(self.pos, self.mass, self.acc)
             if self.check_finite:
                 self._check_bh()
+        elif self.pos_lo is not None:
+            gravity.compute_acc_direct_hilo(self.pos, self.pos_lo, self.mass, self.acc, self.n,
+                                            self.G, self.softening ** 2)
         else:
             gravity.compute_acc_direct(self.pos, self.mass, self.acc, self.n, self.G,
                                        self.softening ** 2, self.precision)
@@ -163,6 +173,12 @@ class Engine:
             self._bh.step(self.pos, self.vel, self.mass, self.acc, half, dt)
             if self.check_finite:
                 self._check_bh()
+        elif self.pos_lo is not None:
+            # hi/lo positions: compensated drift, FP32C force on both parts
+            leapfrog.kick_drift_hilo(self.pos, self.pos_lo, self.vel, self.acc, self.n, half, dt)
+            gravity.compute_acc_direct_kick_hilo(self.pos, self.pos_lo, self.mass, self.acc,
+                                                 self.vel, self.n, self.G, self.softening ** 2,
+                                                 half)
         else:
             leapfrog.kick_drift(self.pos, self.vel, self.acc, self.n, half, dt)
             gravity.compute_acc_direct_kick(self.pos, self.mass, self.acc, self.vel, self.n,

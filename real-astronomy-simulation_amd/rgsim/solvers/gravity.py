@@ -16,6 +16,8 @@ and the accumulate forms the gas-disk and living-galaxy engines use
 ``pos``/``acc`` are (n,3) and ``mass`` (n,) contiguous float64 CUDA tensors.
 ``precision`` selects the pair arithmetic: "fp64" (the reference's),
 "fp32", or "fp32c" (FP32 on hi+lo positions with compensated sums).
+``compute_acc_direct_hilo`` / ``compute_acc_direct_kick_hilo`` take the
+positions as FP64 ``pos`` plus FP32 offsets ``pos_lo`` (FP32C arithmetic).
 """
 from __future__ import annotations
 
@@ -24,7 +26,7 @@ import ctypes
 import torch
 
 from .. import _lib
-from .._tensors import vec1, vec3
+from .._tensors import vec1, vec3, vec3_f32
 
 
 def compute_acc_direct(pos, mass, acc, n, grav, eps2, precision="fp64"):
@@ -48,6 +50,31 @@ def compute_acc_direct_kick(pos, mass, acc, vel, n, grav, eps2, dt_half, precisi
                                               _lib.precision_code(precision), float(dt_half),
                                               _lib.stream_ptr(pos)),
                "rg_direct_acc_kick")
+
+
+def compute_acc_direct_hilo(pos, pos_lo, mass, acc, n, grav, eps2):
+    """compute_acc_direct in FP32C on hi/lo positions X = pos + pos_lo."""
+    n = int(n)
+    vec3(pos, n, "pos"), vec3_f32(pos_lo, n, "pos_lo"), vec1(mass, n, "mass")
+    vec3(acc, n, "acc")
+    _lib.require_gpu()
+    _lib.check(_lib.load().rg_direct_acc_hilo(pos.data_ptr(), pos_lo.data_ptr(), mass.data_ptr(),
+                                              acc.data_ptr(), n, float(grav), float(eps2),
+                                              _lib.stream_ptr(pos)),
+               "rg_direct_acc_hilo")
+
+
+def compute_acc_direct_kick_hilo(pos, pos_lo, mass, acc, vel, n, grav, eps2, dt_half):
+    """compute_acc_direct_hilo followed by v += acc * dt_half in the same kernel."""
+    n = int(n)
+    vec3(pos, n, "pos"), vec3_f32(pos_lo, n, "pos_lo"), vec1(mass, n, "mass")
+    vec3(acc, n, "acc"), vec3(vel, n, "vel")
+    _lib.require_gpu()
+    _lib.check(_lib.load().rg_direct_acc_kick_hilo(pos.data_ptr(), pos_lo.data_ptr(),
+                                                   mass.data_ptr(), acc.data_ptr(),
+                                                   vel.data_ptr(), n, float(grav), float(eps2),
+                                                   float(dt_half), _lib.stream_ptr(pos)),
+               "rg_direct_acc_kick_hilo")
 
 
 def potential_struct(M_d, a, b, M_h, a_h, is_hernquist, smbh_mass):

@@ -17,6 +17,11 @@ Both are lossless for the float64 fields, so a run resumed from a snapshot
 continues bit for bit; the Engine stores its solver settings (precision,
 tree parameters) in ``meta`` so that ``Engine.from_state`` rebuilds the same
 solver.
+
+``pos_lo`` (optional, (N,3) float32; not a field of the reference's State)
+holds the FP32 offsets of an FP32C engine's hi/lo positions, X = pos +
+pos_lo; ``pos`` alone is X rounded to FP64.  Both snapshot formats keep it,
+so an FP32C run resumes bit for bit too.
 """
 from __future__ import annotations
 
@@ -49,6 +54,7 @@ class State:
     u: Optional[np.ndarray] = None
     rho: Optional[np.ndarray] = None
     meta: dict = field(default_factory=dict)
+    pos_lo: Optional[np.ndarray] = None
 
     @property
     def n(self) -> int:
@@ -60,6 +66,10 @@ class State:
         self.mass = np.ascontiguousarray(self.mass, dtype=np.float64).reshape(-1)
         self.ptype = np.ascontiguousarray(self.ptype, dtype=np.int32).reshape(-1)
         self.ids = np.ascontiguousarray(self.ids, dtype=np.int64).reshape(-1)
+        if self.pos_lo is not None:
+            self.pos_lo = np.ascontiguousarray(self.pos_lo, dtype=np.float32).reshape(-1, 3)
+            if self.pos_lo.shape != self.pos.shape:
+                raise ValueError(f"pos_lo must have pos's shape {self.pos.shape}")
 
     # ------------------------------------------------------------- HDF5
     def save(self, path: str) -> None:
@@ -82,6 +92,8 @@ class State:
                 if arr is not None:
                     f.create_dataset(name, data=np.asarray(arr, dtype=np.float64),
                                      compression="gzip")
+            if self.pos_lo is not None:
+                f.create_dataset("pos_lo", data=self.pos_lo, compression="gzip")
 
     @classmethod
     def load(cls, path: str) -> "State":
@@ -90,7 +102,7 @@ class State:
 
         with h5py.File(path, "r") as f:
             meta = {k[5:]: _plain(f.attrs[k]) for k in f.attrs if k.startswith("meta_")}
-            opt = {name: f[name][:] for name in OPTIONAL if name in f}
+            opt = {name: f[name][:] for name in OPTIONAL + ("pos_lo",) if name in f}
             return cls(pos=f["pos"][:], vel=f["vel"][:], mass=f["mass"][:],
                        ptype=f["ptype"][:], ids=f["ids"][:], time=float(f.attrs["time"]),
                        step=int(f.attrs["step"]), seed=int(f.attrs["seed"]), meta=meta, **opt)
@@ -98,7 +110,8 @@ class State:
     # -------------------------------------------------------------- npz
     def save_npz(self, path: str) -> None:
         """The same fields with numpy only; meta as a JSON string."""
-        extra = {k: getattr(self, k) for k in OPTIONAL if getattr(self, k) is not None}
+        extra = {k: getattr(self, k) for k in OPTIONAL + ("pos_lo",)
+                 if getattr(self, k) is not None}
         np.savez(path, pos=self.pos, vel=self.vel, mass=self.mass, ptype=self.ptype,
                  ids=self.ids, time=self.time, step=self.step, seed=self.seed,
                  meta=np.array(json.dumps(self.meta, default=str)), **extra)
@@ -106,7 +119,7 @@ class State:
     @classmethod
     def load_npz(cls, path: str) -> "State":
         with np.load(path, allow_pickle=False) as f:
-            opt = {k: f[k] for k in OPTIONAL if k in f}
+            opt = {k: f[k] for k in OPTIONAL + ("pos_lo",) if k in f}
             meta = json.loads(str(f["meta"])) if "meta" in f else {}
             return cls(pos=f["pos"], vel=f["vel"], mass=f["mass"], ptype=f["ptype"],
                        ids=f["ids"], time=float(f["time"]), step=int(f["step"]),

@@ -181,3 +181,6 @@ def test_resume_from_snapshot_keeps_solver(tmp_path, mode, precision):
     sa, sc = a.to_state(), c.to_state()
     assert np.array_equal(sa.pos, sc.pos) and np.array_equal(sa.vel, sc.vel)
     assert c.step_count == 12 and sc.meta["engine_precision"] == precision
+    if precision == "fp32c":  # the hi/lo offsets travel with the snapshot
+        assert sc.pos_lo is not None and np.array_equal(sa.pos_lo, sc.pos_lo)
+        assert np.any(sc.pos_lo != 0)

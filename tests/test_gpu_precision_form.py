@@ -3,7 +3,8 @@ eps 0.005, seed 0).  FP32C pair sums (FP32 on hi/lo positions, compensated
 partials) against the FP64 sum in the reference's order (gravity.py:13-26,
 bit-exact with the oracle at every size the oracle tests run), held to the
 1e-6 maximum relative force error the form is specified with; then 50 KDK
-steps at dt 5e-4 (engine.py:108-116) with FP64 energy and momentum.  The
+steps at dt 5e-4 (engine.py:108-116) with FP64-hi + FP32-lo positions in
+the drift, against the FP64 engine's trajectory and energy drift.  The
 plain-FP32 variant of the same configuration is held to its own stated
 bound."""
 import numpy as np
@@ -31,17 +32,33 @@ def test_fp32c_full_size_error_and_kdk():
     print(f"  fp32c max {rel.max():.2e} rms {np.sqrt(np.mean(rel ** 2)):.2e}", flush=True)
     assert rel.max() <= 1e-6 and np.sqrt(np.mean(rel ** 2)) <= 1e-7
 
+    # 50 KDK steps with hi/lo positions (compensated drift, FP32C forces on
+    # both parts) beside the FP64 engine (the reference's arithmetic, whose
+    # trajectory is the oracle's bit for bit at every size the oracle runs)
     eng = Engine(softening=0.005, precision="fp32c", G=1.0)
     eng.load_state(s)
+    e64 = Engine(softening=0.005, precision="fp64", G=1.0)
+    e64.load_state(s)
+    assert eng.pos_lo is not None and e64.pos_lo is None
     _, _, e0 = eng.energies()
     p0 = np.array(eng.momentum())
     for _ in range(50):
         eng.step(5e-4)
+        e64.step(5e-4)
     ke1, pe1, e1 = eng.energies()
     p1 = np.array(eng.momentum())
+    _, _, f1 = e64.energies()
     drift = abs((e1 - e0) / e0)
-    print(f"  50 KDK steps: |dE/E| {drift:.2e}, virial {-2 * ke1 / pe1:.4f}", flush=True)
-    assert drift < 1e-8
+    drift64 = abs((f1 - e0) / e0)
+    out, ref64 = eng.to_state(), e64.to_state()
+    x = out.pos + out.pos_lo.astype(np.float64)
+    dx = np.abs(x - ref64.pos).max() / np.abs(ref64.pos).max()
+    dv = np.abs(out.vel - ref64.vel).max() / np.abs(ref64.vel).max()
+    print(f"  50 KDK steps: |dE/E| {drift:.2e} (FP64 engine {drift64:.2e}), virial "
+          f"{-2 * ke1 / pe1:.4f}, max |dx| {dx:.2e}, max |dv| {dv:.2e} of the scale",
+          flush=True)
+    assert drift < 1e-8 and drift <= 1.05 * drift64 + 1e-12
+    assert dx <= 1e-12 and dv <= 1e-8
     assert 0.8 < -2.0 * ke1 / pe1 < 1.2
     assert np.linalg.norm(p1 - p0) < 1e-12
 
