@@ -30,14 +30,19 @@ def test_engine_settings_round_trip_through_npz(tmp_path):
     rng = np.random.default_rng(3)
     s = State(pos=rng.normal(size=(9, 3)), vel=rng.normal(size=(9, 3)), mass=np.ones(9),
               ptype=np.zeros(9), ids=np.arange(9), time=1.5, step=7, seed=4,
-              age=rng.random(9), meta={"level": "medium", **e.settings()})
+              age=rng.random(9), meta={"level": "medium", **e.settings()},
+              pos_lo=rng.normal(size=(9, 3)) * 1e-17)
+    assert s.pos_lo.dtype == np.float32
     path = str(tmp_path / "s.npz")
     s.save_npz(path)
     t = State.load_npz(path)
     assert t.meta == s.meta and t.step == 7 and t.time == 1.5 and t.seed == 4
-    for name in ("pos", "vel", "mass", "ptype", "ids", "age"):
+    for name in ("pos", "vel", "mass", "ptype", "ids", "age", "pos_lo"):
         assert np.array_equal(getattr(t, name), getattr(s, name))
     assert t.temperature is None
+    with pytest.raises(ValueError):
+        State(pos=np.zeros((2, 3)), vel=np.zeros((2, 3)), mass=np.ones(2), ptype=np.zeros(2),
+              ids=np.arange(2), pos_lo=np.zeros((3, 3)))
 
 
 def test_hdf5_snapshot_needs_h5py(tmp_path):
