@@ -46,7 +46,8 @@ WALK_RECORD_BYTES = 32          # FP32 node record fetched per visit
 # FP32 walk variant (rg_bh_stats walk_mode) -> the kernel it launches
 WALK_KERNELS = {0: "k_walk32", 1: "k_walk32_pair", 2: "k_walk32_wave", 3: "k_walk32_wave2",
                 4: "k_walk32_wave_split<2>", 5: "k_walk32_wave_split<4>",
-                6: "k_walk32_chunked<{w}>", 7: "k_walk32_chunked<{w}, true>", -1: "k_walk64"}
+                6: "k_walk32_chunked<{w}>", 7: "k_walk32_chunked<{w}, true>",
+                8: "k_walk32_chunked2<{w}>", -1: "k_walk64"}
 WALK_PARTICLE_BYTES = 16 + 4 + 24  # own leaf record + order index + FP64 acceleration
 HILO_KD_BYTES = (24 + 12 + 24 + 24) + (24 + 12 + 24)  # hi/lo kick + drift, read + written
 WALK_NODE_BYTES = 16 + 8           # FP32 record pair (com, G m) + (threshold, skip), per node

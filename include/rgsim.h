@@ -92,13 +92,15 @@ const char *rg_last_error(void);
  *                 the wave walk split over two / four waves per particle
  *                 group, 6 the wave walk over far-field chunks of the
  *                 preorder that the group's waves take in turn, 7 walk 6
- *                 with its record window in LDS (same results as 6, bitwise)
+ *                 with its record window in LDS (same results as 6, bitwise),
+ *                 8 walk 7 with two chunks in flight per wave (same results)
  *   "chunk_dmin", "chunk_shift", "chunk_run", "chunk_waves"
  *                 shape of walks 6 and 7: leaf distance of the nearest chunk
  *                 boundary beyond each end of a group (default 256); growth
  *                 factor 2^shift of the next ones (0 = by n: 1 below 350k
  *                 particles, else 2); runs of groups per XCD (32); waves per
- *                 group, 4 or 8 (0 = by n: 8 below 90k particles)
+ *                 group, 4 or 8, or 2 for walk 8 (0 = by n: 8 below 90k
+ *                 particles)
  * Process-wide; takes effect at the next launch. */
 int rg_set_param(const char *name, int64_t value);
 
@@ -305,7 +307,7 @@ typedef struct {
     int64_t visits_total;   /* nodes examined by the walk, all particles */
     int64_t visits_max;     /* max nodes examined by one particle */
     int64_t nonfinite;      /* non-finite input coordinates seen */
-    int64_t walk_mode;      /* FP32 walk variant used (0..7, see "walk_mode"); -1 FP64 */
+    int64_t walk_mode;      /* FP32 walk variant used (0..8, see "walk_mode"); -1 FP64 */
     int64_t walk_waves;     /* waves per group of 64 particles in it (0: one lane each) */
 } rg_bh_stats_t;
 

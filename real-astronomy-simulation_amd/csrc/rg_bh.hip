@@ -1696,6 +1696,256 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     }
 }
 
+
+// The chunked wave walk (k_walk32_chunked, record window in LDS) with two
+// chunk walks in flight per wave.  A step of the wave walk is one chain of
+// dependent operations -- the slot's skip (v_readlane), its record (LDS),
+// the criterion, the ballot that picks the next slot -- so a wave with one
+// walk mostly waits.  Here each wave holds walks A and B on two chunks and
+// advances both in one straight-line step: both records are read before
+// either is tested, so one walk's LDS and ballot latency hides behind the
+// other's arithmetic.  The ancestor and own phases of a chunk are the same
+// code here (a scalar flag per step): before the chunk's first slot a node
+// is only tested (subtrees ending at or before it are stepped over), from
+// it on every node a lane reaches is counted and, when accepted, added.  Per
+// lane and chunk the nodes, their order and the arithmetic are those of
+// k_walk32_chunked, and the chunk partials are added in slot order at the
+// end, so the accelerations are the same floats.
+struct ChunkWalk {
+    int c, lo, hi;  // chunk and its slot range (uniform)
+    int k, k0;      // current slot and window base (uniform)
+    int next;       // per lane: first slot this lane takes part in again
+    int skipw;      // per lane: skip slot of window record k0 + lane
+    float ax, ay, az;
+    int visits;
+};
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_walk32_chunked2(
+    const float4 *__restrict__ recA, const int2 *__restrict__ recB, const int *__restrict__ pre,
+    const int *__restrict__ order, AccOut out, int n, float eps2,
+    unsigned long long *__restrict__ stats, ChunkShape cs)
+{
+    if (tree_aborted(stats)) return;
+    __shared__ int s_b[kChunkMaxB + 1];
+    __shared__ int s_nc;
+    __shared__ int s_ctr;
+    __shared__ int s_done;
+    extern __shared__ float s_part_dyn[];
+    float(*s_part)[3][64] = reinterpret_cast<float(*)[3][64]>(s_part_dyn);
+    __shared__ int s_vis[64];
+    // per wave and walk: the window's rows (com x, y, z, G m, threshold)
+    __shared__ float s_win[NW][2][5][64];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int g = xcd_run_block(blockIdx.x, gridDim.x, cs.run);
+    const int t0 = g * 64;
+    const int t = t0 + lane;
+    const bool active = t < n;
+    const int m = 2 * n - 1;
+    const int nact = min(64, n - t0);
+    const int last = t0 + nact - 1;
+    int nr = 0, nl = 0;
+    for (long long d = cs.dmin; last + d <= n - 1 && nr < kChunkSide; d <<= cs.shift) ++nr;
+    for (long long d = cs.dmin; t0 - d >= 0 && nl < kChunkSide; d <<= cs.shift) ++nl;
+    const int ni = nact > 33 ? 1 : 0;
+    const int nbd = nr + ni + nl;
+    if (wave == 0) {
+        if (lane < nbd) {
+            long long leaf;
+            if (lane < nr)
+                leaf = last + (static_cast<long long>(cs.dmin) << (cs.shift * (nr - 1 - lane)));
+            else if (lane < nr + ni)
+                leaf = t0 + 32;
+            else
+                leaf = t0 - (static_cast<long long>(cs.dmin) << (cs.shift * (lane - nr - ni)));
+            s_b[lane + 1] = pre[leaf];
+        }
+        if (lane == 0) {
+            s_b[0] = 0;
+            s_b[nbd + 1] = m;
+            s_nc = nbd + 1;
+            s_ctr = 0;
+            s_done = 0;
+        }
+        s_vis[lane] = 0;
+    }
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (active) {
+        const float4 self = recA[pre[t]];
+        px = self.x;
+        py = self.y;
+        pz = self.z;
+    }
+    __syncthreads();
+    const int nc = s_nc;
+    int wvis = 0;
+
+    // Store a finished chunk's partial sums and visits.
+    auto finish = [&](ChunkWalk &w) {
+        s_part[w.c][0][lane] = w.ax;
+        s_part[w.c][1][lane] = w.ay;
+        s_part[w.c][2][lane] = w.az;
+        if (w.visits) atomicAdd(&s_vis[lane], w.visits);
+        wvis += w.visits;
+    };
+    // Take the next non-empty chunk (farthest first, alternating ends);
+    // false when the block has none left.
+    auto fetch = [&](ChunkWalk &w) -> bool {
+        while (true) {
+            int j = 0;
+            if (lane == 0) j = atomicAdd(&s_ctr, 1);
+            j = __builtin_amdgcn_readfirstlane(__shfl(j, 0, 64));
+            if (j >= nc) return false;
+            w.c = (j & 1) ? nc - 1 - (j >> 1) : (j >> 1);
+            w.lo = s_b[w.c];
+            w.hi = s_b[w.c + 1];
+            w.ax = w.ay = w.az = 0.f;
+            w.visits = 0;
+            w.next = (active && w.lo < w.hi) ? 0 : m;
+            w.k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(w.next));
+            w.k0 = -64;
+            if (w.k < w.hi) return true;
+            finish(w);  // empty: its partials are zero
+        }
+    };
+    auto refill = [&](ChunkWalk &w, int b) {
+        if (w.k - w.k0 >= 64) {
+            w.k0 = w.k;
+            const int slot = w.k0 + lane;
+            if (slot < m) {
+                const float4 a = recA[slot];
+                const int2 r = recB[slot];
+                float *row = &s_win[wave][b][0][lane];
+                row[0] = a.x;
+                row[64] = a.y;
+                row[128] = a.z;
+                row[192] = a.w;
+                row[256] = __int_as_float(r.x);
+                w.skipw = r.y;
+            }
+            // the window is read across lanes: keep the reads after the writes
+            __builtin_amdgcn_wave_barrier();
+        }
+    };
+    // One step at slot w.k, in three parts so that two walks interleave:
+    // load (skip and record), test and accumulate, advance.
+    struct Rec {
+        int skip;
+        float cx, cy, cz, cw, crit;
+    };
+    auto load = [&](const ChunkWalk &w, int b) -> Rec {
+        const int l = w.k - w.k0;
+        Rec r;
+        r.skip = max(__builtin_amdgcn_readlane(w.skipw, l), w.k + 1);
+        const float *row = &s_win[wave][b][0][l];
+        r.cx = row[0];
+        r.cy = row[64];
+        r.cz = row[128];
+        r.cw = row[192];
+        r.crit = row[256];
+        return r;
+    };
+    auto test = [&](ChunkWalk &w, const Rec &r) {
+        const bool own = w.k >= w.lo;             // uniform
+        const bool over = !own && r.skip <= w.lo;  // uniform: subtree ends before the chunk
+        const bool part = w.next <= w.k;
+        const float dx = r.cx - px;
+        const float dy = r.cy - py;
+        const float dz = r.cz - pz;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+        const bool acc = r.crit < r2;
+        if (own && part) ++w.visits;
+        if (part && (over || acc)) w.next = r.skip;
+        if (own && part && acc) {
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float sc = r.cw * (inv * inv) * inv;
+            w.ax = fmaf(sc, dx, w.ax);
+            w.ay = fmaf(sc, dy, w.ay);
+            w.az = fmaf(sc, dz, w.az);
+        }
+    };
+    auto advance = [&](ChunkWalk &w, const Rec &r) {
+        // k + 1 if a lane opened k, else skip(k): every other lane waits at
+        // an ancestor's skip, >= skip(k)
+        w.k = __ballot(w.next <= w.k + 1) ? w.k + 1 : r.skip;
+    };
+
+    ChunkWalk A, B;
+    bool aliveA = fetch(A);
+    bool aliveB = aliveA && fetch(B);
+    while (aliveA && aliveB) {
+        refill(A, 0);
+        refill(B, 1);
+        const Rec ra = load(A, 0);
+        const Rec rb = load(B, 1);
+        test(A, ra);
+        test(B, rb);
+        advance(A, ra);
+        advance(B, rb);
+        if (A.k >= A.hi) {
+            finish(A);
+            aliveA = fetch(A);
+        }
+        if (B.k >= B.hi) {
+            finish(B);
+            aliveB = fetch(B);
+        }
+    }
+    // the block's chunks are all taken: finish the walk still in flight
+    if (aliveB) {
+        A = B;
+        aliveA = true;
+        // its window sits in row set 1
+        while (aliveA) {
+            refill(A, 1);
+            const Rec ra = load(A, 1);
+            test(A, ra);
+            advance(A, ra);
+            if (A.k >= A.hi) {
+                finish(A);
+                aliveA = fetch(A);
+            }
+        }
+    }
+    while (aliveA) {
+        refill(A, 0);
+        const Rec ra = load(A, 0);
+        test(A, ra);
+        advance(A, ra);
+        if (A.k >= A.hi) {
+            finish(A);
+            aliveA = fetch(A);
+        }
+    }
+    {
+        const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(wvis));
+        if (lane == 0) atomicAdd(&stats[0], tot);
+    }
+    // As k_walk32_chunked: no block barrier; the last wave to finish adds
+    // the chunk partials in slot order.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    int done = 0;
+    if (lane == 0) done = atomicAdd(&s_done, 1);
+    done = __builtin_amdgcn_readfirstlane(__shfl(done, 0, 64));
+    if (done == NW - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        for (int c = 0; c < nc; ++c) {
+            ax += s_part[c][0][lane];
+            ay += s_part[c][1][lane];
+            az += s_part[c][2][lane];
+        }
+        if (active) {
+            const int p = order[t];
+            store_acc(out, p, static_cast<double>(ax), static_cast<double>(ay),
+                      static_cast<double>(az));
+        }
+        const int mx = wave_max_i32(s_vis[lane]);
+        if (lane == 0) atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+    }
+}
+
 }  // namespace
 
 // ================================================================ handle
@@ -1776,7 +2026,7 @@ int bh_mark(rg_bh *h, int phase, hipStream_t st)
 
 int rg::rg_bh_set_walk_mode_impl(int64_t v)
 {
-    RG_REQUIRE(v >= -1 && v <= 7, RG_ERR_ARG, "walk_mode must be -1 (auto) or 0..7");
+    RG_REQUIRE(v >= -1 && v <= 8, RG_ERR_ARG, "walk_mode must be -1 (auto) or 0..8");
     g_walk_mode = static_cast<int>(v);
     return RG_OK;
 }
@@ -1793,7 +2043,8 @@ int rg::rg_bh_set_chunk_impl(const char *what, int64_t v)
         RG_REQUIRE(v >= 1 && v <= 4096, RG_ERR_ARG, "chunk_run must be in [1, 4096]");
         g_chunk_run = static_cast<int>(v);
     } else if (std::strcmp(what, "waves") == 0) {
-        RG_REQUIRE(v == 0 || v == 4 || v == 8, RG_ERR_ARG, "chunk_waves must be 0 (auto), 4 or 8");
+        RG_REQUIRE(v == 0 || v == 2 || v == 4 || v == 8, RG_ERR_ARG,
+                   "chunk_waves must be 0 (auto), 2 (walk 8 only), 4 or 8");
         g_chunk_waves = static_cast<int>(v);
     } else {
         rg::set_error("unknown parameter 'chunk_%s'", what);
@@ -2029,13 +2280,14 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
         // reads its record with one LDS broadcast load.
         const int mode = g_walk_mode >= 0 ? g_walk_mode : 7;
         h->walk_used = mode;
-        if (mode >= 6) {
+        if (mode >= 6) {  // 6, 7 and 8 (the chunked walks)
             // Measured on disk + bulge galaxies, 60k .. 2M particles: octave
             // chunks (growth 2) below ~350k, then growth 4 (fewer ancestor
             // tests once the walk is throughput-bound); 8 waves per group only
             // pay below ~90k, where the groups are too few to fill the GPU.
             const int shift = g_chunk_shift ? g_chunk_shift : (n < 350000 ? 1 : 2);
-            const int waves = g_chunk_waves ? g_chunk_waves : (n < 90000 ? 8 : 4);
+            int waves = g_chunk_waves ? g_chunk_waves : (n < 90000 ? 8 : 4);
+            if (mode != 8 && waves == 2) waves = 4;  // two waves only for the two-walk kernel
             const ChunkShape cs{g_chunk_dmin, shift, g_chunk_run};
             const int ng = (n + 63) / 64;
             h->walk_waves = waves;
@@ -2046,7 +2298,16 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
             const int ncmax = 2 * side + 2;
             const size_t smem = sizeof(float) * 3 * 64 * static_cast<size_t>(ncmax);
             const float e2 = static_cast<float>(eps2);
-            if (mode == 7 && waves == 8)
+            if (mode == 8 && waves == 8)
+                k_walk32_chunked2<8><<<ng, 512, smem, st>>>(h->recA, h->recB, h->pre, order, out,
+                                                            n, e2, h->stats, cs);
+            else if (mode == 8 && waves == 2)
+                k_walk32_chunked2<2><<<ng, 128, smem, st>>>(h->recA, h->recB, h->pre, order, out,
+                                                            n, e2, h->stats, cs);
+            else if (mode == 8)
+                k_walk32_chunked2<4><<<ng, 256, smem, st>>>(h->recA, h->recB, h->pre, order, out,
+                                                            n, e2, h->stats, cs);
+            else if (mode == 7 && waves == 8)
                 k_walk32_chunked<8, true><<<ng, 512, smem, st>>>(h->recA, h->recB, h->pre, order,
                                                                  out, n, e2, h->stats, cs);
             else if (mode == 7)

@@ -196,7 +196,7 @@ def test_fp32_walk_modes_agree(name):
     from rgsim import _lib
     pos, mass = CASES[name]
     res = {}
-    for mode in (0, 1, 2, 3, 4, 5, 6, 7):
+    for mode in (0, 1, 2, 3, 4, 5, 6, 7, 8):
         _lib.set_param("walk_mode", mode)
         try:
             bh, acc = run_bh(pos, mass, theta=0.5, eps=0.05, precision="fp32")
@@ -204,7 +204,7 @@ def test_fp32_walk_modes_agree(name):
         finally:
             _lib.set_param("walk_mode", -1)
     (a0, s0) = res[0]
-    for mode in (1, 2, 3, 4, 5, 6, 7):
+    for mode in (1, 2, 3, 4, 5, 6, 7, 8):
         s = res[mode][1]
         assert s["walk_mode"] == mode
         assert s0["visits_total"] == s["visits_total"] and s0["visits_max"] == s["visits_max"], mode
@@ -222,7 +222,7 @@ def test_fp32_walk_modes_agree(name):
         assert np.abs(a0 - res[mode][0]).max() <= 1e-5 * scale, mode
     for mode in (2, 3):  # each lane's summation order unchanged
         assert np.array_equal(a0, res[mode][0]), mode
-    for mode in (7,):  # the same walk, records read from LDS
+    for mode in (7, 8):  # the same walk: records read from LDS; two chunks per wave at once
         assert np.array_equal(res[6][0], res[mode][0]), mode
 
 
