@@ -13,11 +13,24 @@ keyword ``G`` with that default.  ``precision`` selects the self-gravity pair
 arithmetic (FP64 by default, the reference's); the analytic terms are FP64.
 The tree path of ``_forces`` (:218-219, BarnesHut.compute into acc_g, then
 add_field) is ``BarnesHut.compute_add`` in one call.
+
+``GasDiskGravity`` is the engine's whole gravity half -- the tree rule and
+n-change rebuild (:169-173) and the adds of ``_forces`` (:216-230): the tree
+then the analytic potential, or the fused N^2 + analytic kernel.
 """
 from __future__ import annotations
 
 from .solvers import gravity as _g
+from .solvers.self_gravity import GAS_DISK, SelfGravity
 from .units import G as _G
+
+
+class GasDiskGravity(SelfGravity):
+    """SelfGravity with the gas-disk engine's order of adds: acc + (((self +
+    disk) + halo) + point mass) in one kernel on the direct path."""
+
+    def __init__(self, softening, gravity_mode="direct", theta=0.6, pot=None, **kw):
+        super().__init__(softening, gravity_mode, theta, pot, flavor=GAS_DISK, **kw)
 
 
 def add_grav_ext(pos, mass, acc, n, eps2, M_d, a, b, M_h, a_h, is_hernquist, smbh_mass, *,

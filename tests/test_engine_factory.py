@@ -59,3 +59,19 @@ def test_hdf5_snapshot_needs_h5py(tmp_path):
     s.save(str(tmp_path / "s.h5"))
     t = State.load(str(tmp_path / "s.h5"))
     assert t.meta == s.meta and np.array_equal(t.mass, s.mass)
+
+
+def test_engine_gravity_rules_without_gpu():
+    """The engines' tree rule (living_galaxy_engine.py:734, gas_disk_engine.py:
+    169): a tree iff gravity_mode == "bh" and n >= 256; bad settings raise
+    before any device work."""
+    from rgsim.gas_disk_engine import GasDiskGravity
+    from rgsim.living_galaxy_engine import LivingGalaxyGravity
+    from rgsim.solvers.self_gravity import SelfGravity, uses_tree
+    assert uses_tree("bh", 256) and not uses_tree("bh", 255) and not uses_tree("direct", 10 ** 6)
+    assert LivingGalaxyGravity(0.05).flavor == "living_galaxy"
+    assert GasDiskGravity(0.05, "bh", 0.4, {"M_d": 5.0}).pot.M_d == 5.0
+    with pytest.raises(ValueError):
+        SelfGravity(0.05, flavor="sph")
+    with pytest.raises(ValueError):
+        SelfGravity(0.05, gravity_mode="tree")
