@@ -73,10 +73,14 @@ const char *rg_last_error(void);
  *                 treelet ranking) makes before it gives up and flags the
  *                 failure: -1 restores the default 2^20, 0 gives up at once
  *                 (to test the failure path; every kernel still exits)
- *   "sort_items"  keys per thread in a radix-sort tile, for BH and SPH-grid
- *                 handles created afterwards: 0 (default: 4 below 524,288
- *                 keys, 8 below 786,432, else 16), 4, 8 or 16.  Same results
- *                 either way
+ *   "sort_block"  threads per radix-sort tile, for BH and SPH-grid handles
+ *                 created afterwards: 0 (default: 1024 from 200,000 keys,
+ *                 else 256), 256, or 1024 (with 4 keys per thread, whatever
+ *                 "sort_items" says).  Same results either way
+ *   "sort_items"  keys per thread in a 256-thread radix-sort tile, for BH and
+ *                 SPH-grid handles created afterwards: 0 (default: 4 below
+ *                 524,288 keys, 8 below 786,432, else 16), 4, 8 or 16.  Same
+ *                 results either way
  *   "direct_split" j-slices per i-block of the FP32/FP32C direct sums and of
  *                 rg_direct_vjp: 0 (default: 32 with one i-particle per
  *                 lane, 64 with more, fewer when slices x n would pass 2^24)

@@ -1957,6 +1957,7 @@ struct rg_bh {
     int device = 0;
     int nblk = 0;           // sort tiles
     int sort_items = 0;     // keys per thread in a sort tile
+    int sort_bs = 256;      // threads per sort tile
     int passes = 0;         // radix passes
     int final_buf = 0;      // ping-pong buffer holding the sorted result
     hipStream_t last_stream = nullptr;
@@ -2089,8 +2090,10 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     }
     const size_t N = static_cast<size_t>(n);
     const size_t M = 2 * N - 1;
-    h->sort_items = sort_items_for(static_cast<int64_t>(N));
-    h->nblk = sort_tiles(static_cast<int64_t>(N), h->sort_items);
+    h->sort_bs = sort_block_for(static_cast<int64_t>(N));
+    h->sort_items = h->sort_bs == kSortBSWide ? kSortItemsSmall
+                                              : sort_items_for(static_cast<int64_t>(N));
+    h->nblk = sort_tiles(static_cast<int64_t>(N), h->sort_items, h->sort_bs);
     h->passes = (3 * morton_bits + kDigitBits - 1) / kDigitBits;
     int st = RG_OK;
 #define ALLOC(ptr, bytes)                                                  \
@@ -2203,7 +2206,7 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     // Stable LSD radix sort, kDigitBits-bit digits, one kernel per pass.
     int cur = 0;
     for (int p = 0; p < h->passes; ++p) {
-        launch_onesweep(h->sort_items, h->nblk, st, h->keys[cur], p == 0 ? nullptr : h->idx[cur],
+        launch_onesweep(h->sort_items, h->sort_bs, h->nblk, st, h->keys[cur], p == 0 ? nullptr : h->idx[cur],
                         h->keys[cur ^ 1], h->idx[cur ^ 1], n, kDigitBits * p,
                         h->ghist + kBuckets * p,
                         h->lookback + static_cast<size_t>(kBuckets) * h->nblk * p,

@@ -24,6 +24,7 @@ namespace rg {
 
 int g_sort_ticket = 0;
 int g_sort_items = 0;
+int g_sort_block = 0;
 int g_spin_limit = kDefaultSpinLimit;
 
 void set_error(const char *fmt, ...)
@@ -83,6 +84,12 @@ extern "C" int rg_set_param(const char *name, int64_t value)
         RG_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16, RG_ERR_ARG,
                    "sort_items must be 0, 4, 8 or 16");
         rg::g_sort_items = static_cast<int>(value);
+        return RG_OK;
+    }
+    if (std::strcmp(name, "sort_block") == 0) {
+        RG_REQUIRE(value == 0 || value == 256 || value == 1024, RG_ERR_ARG,
+                   "sort_block must be 0, 256 or 1024");
+        rg::g_sort_block = static_cast<int>(value);
         return RG_OK;
     }
     if (std::strcmp(name, "sort_spin_limit") == 0) {

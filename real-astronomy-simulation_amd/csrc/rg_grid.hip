@@ -250,6 +250,7 @@ struct rg_grid {
     int passes = 0;
     int nblk = 0;  // sort tiles
     int sort_items = 0;  // keys per thread in a sort tile
+    int sort_bs = 256;      // threads per sort tile
     int npart = 0; // reduction blocks
     uint64_t *keys[2] = {nullptr, nullptr};
     int *idx[2] = {nullptr, nullptr};
@@ -296,8 +297,9 @@ extern "C" int rg_grid_create(int64_t n, double min_radius, rg_grid **out)
     while ((1ll << bits) <= g->max_cells) ++bits;  // keys are <= max_cells
     g->passes = (bits + kDigitBits - 1) / kDigitBits;
     const int64_t N = n > 0 ? n : 1;
-    g->sort_items = sort_items_for(N);
-    g->nblk = sort_tiles(N, g->sort_items);
+    g->sort_bs = sort_block_for(N);
+    g->sort_items = g->sort_bs == kSortBSWide ? kSortItemsSmall : sort_items_for(N);
+    g->nblk = sort_tiles(N, g->sort_items, g->sort_bs);
     g->npart = static_cast<int>(std::min<int64_t>((N + kGridBS - 1) / kGridBS, 1024));
     int st = RG_OK;
     RG_HIP(hipGetDevice(&g->device));
@@ -360,7 +362,7 @@ extern "C" int rg_grid_rebuild(rg_grid *g, const double *pos, const double *h,
     RG_CHECK_LAUNCH();
     int cur = 0;
     for (int p = 0; p < g->passes; ++p) {
-        launch_onesweep(g->sort_items, g->nblk, st, g->keys[cur], p == 0 ? nullptr : g->idx[cur],
+        launch_onesweep(g->sort_items, g->sort_bs, g->nblk, st, g->keys[cur], p == 0 ? nullptr : g->idx[cur],
                         g->keys[cur ^ 1], g->idx[cur ^ 1], n, kDigitBits * p,
                         g->ghist + kBuckets * p,
                         g->lookback + static_cast<size_t>(kBuckets) * g->nblk * p,

@@ -33,6 +33,10 @@ extern int g_sort_ticket;
 // (by n), 4, 8 or 16.
 extern int g_sort_items;
 
+// Threads per radix-sort tile for handles created from now on: 0 (by n),
+// 256 or 1024 (with 4 keys per thread).
+extern int g_sort_block;
+
 // Re-polls a decoupled look-back wait (radix sort, treelet ranking) makes
 // before it gives up and flags the failure; 0 gives up at once.
 extern int g_spin_limit;

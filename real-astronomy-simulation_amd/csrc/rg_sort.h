@@ -35,9 +35,24 @@ inline int sort_items_for(int64_t n)
     if (n < kSortLargeFrom) return kSortItemsSmall;
     return n < kSortHugeFrom ? kSortItemsLarge : kSortItemsHuge;
 }
-inline int sort_tiles(int64_t n, int items)
+// Threads per sort tile: 256, or 1024 (16 waves, 4 keys each: a 4096-key
+// tile with the per-thread work of a 256-thread one).  Measured sort phase
+// per BH step (ms, 256-thread default tiles / 1024 x 4): 30k 0.079 / 0.092,
+// 60k 0.090 / 0.100, 120k 0.104 / 0.109, 250k 0.132 / 0.127, 524,288 0.168 /
+// 0.161, 10^6 0.241 (256 x 16, the same 4096-key tiles) / 0.208: below about
+// 200k keys a tile's latency chain grows with its waves more than the
+// shorter look-back saves; above, the 16 waves load and rank a tile's keys
+// in parallel where 256 threads take 8 or 16 rounds.
+constexpr int kSortBSWide = 1024;
+constexpr int64_t kSortWideFrom = 200000;
+inline int sort_block_for(int64_t n)
 {
-    const int64_t tile = static_cast<int64_t>(kSortBS) * items;
+    if (rg::g_sort_block) return rg::g_sort_block;
+    return n >= kSortWideFrom ? kSortBSWide : kSortBS;
+}
+inline int sort_tiles(int64_t n, int items, int bs = kSortBS)
+{
+    const int64_t tile = static_cast<int64_t>(bs) * items;
     return static_cast<int>((n + tile - 1) / tile);
 }
 // Radix digit width.  9-bit digits (7 passes for a 63-bit key) measured
@@ -126,24 +141,30 @@ __device__ __forceinline__ unsigned look_back(const unsigned *__restrict__ lb, i
 // load its keys at once (one memory round trip earlier); that is used only
 // when every tile of the pass fits on the GPU at the same time
 // (kTicketlessTiles), so no tile can wait on one that has no slot.
-template <int ITEMS, bool TICKET>
-__global__ __launch_bounds__(kSortBS) void k_onesweep(
+//
+// BS threads per tile (256 or 1024); threads 0..255 (the first four waves)
+// own one digit each for the scans and the look-back.
+template <int ITEMS, bool TICKET, int BS = kSortBS>
+__global__ __launch_bounds__(BS) void k_onesweep(
     const uint64_t *__restrict__ kin, const int *__restrict__ iin, uint64_t *__restrict__ kout,
     int *__restrict__ iout, int n, int shift, const unsigned *__restrict__ ghist_p,
     unsigned *__restrict__ lookback_p, int *__restrict__ ticket_p,
     unsigned long long *__restrict__ stats, int spin_limit)
 {
-    constexpr int kWaves = kSortBS / 64;
+    static_assert(BS >= kBuckets && BS % 64 == 0, "a tile needs a thread per digit");
+    constexpr int kWaves = BS / 64;
+    constexpr int kOwnWaves = kBuckets / 64;   // waves whose threads own a digit each
     constexpr int kRounds = ITEMS;             // each wave: kRounds rounds of 64 keys
-    constexpr int kSortTile = kSortBS * ITEMS;
-    constexpr int DPT = kBuckets / kSortBS;    // digits owned per thread (adjacent)
+    constexpr int kSortTile = BS * ITEMS;
+    constexpr int DPT = 1;                     // digits owned per owner thread
     __shared__ unsigned s_base[kWaves][kBuckets];
     __shared__ int s_tile;
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int lane = t & 63;
+    const bool owner = t < kBuckets;
     if (TICKET && t == 0) s_tile = atomicAdd(ticket_p, 1);
-    for (int k = t; k < kWaves * kBuckets; k += kSortBS) (&s_base[0][0])[k] = 0u;
+    for (int k = t; k < kWaves * kBuckets; k += BS) (&s_base[0][0])[k] = 0u;
     uint64_t key[kRounds];
     int idv[kRounds];
     auto load_keys = [&](int tl) {
@@ -164,33 +185,33 @@ __global__ __launch_bounds__(kSortBS) void k_onesweep(
     // Exclusive scan of this pass's global digit counts with no block
     // barrier: every wave reads all of them, reduces the ranges of the
     // threads in the waves before it and scans its own range.
-    unsigned gh[DPT];
-    unsigned before = 0, gsum = 0;
+    unsigned gh[DPT] = {0u};
+    unsigned goff[DPT] = {0u};
+    if (w < kOwnWaves) {  // uniform per wave
+        unsigned before = 0, gsum = 0;
 #pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) {
-        unsigned v[DPT];
-        unsigned sum = 0;
+        for (int ww = 0; ww < kOwnWaves; ++ww) {
+            unsigned v[DPT];
+            unsigned sum = 0;
 #pragma unroll
-        for (int q = 0; q < DPT; ++q) {
-            v[q] = ghist_p[(ww * 64 + lane) * DPT + q];
-            sum += v[q];
+            for (int q = 0; q < DPT; ++q) {
+                v[q] = ghist_p[(ww * 64 + lane) * DPT + q];
+                sum += v[q];
+            }
+            if (ww < w) {
+                before += __ockl_wfred_add_u32(sum);
+            } else if (ww == w) {
+                gsum = sum;
+#pragma unroll
+                for (int q = 0; q < DPT; ++q) gh[q] = v[q];
+            }
         }
-        if (ww < w) {
-            before += __ockl_wfred_add_u32(sum);
-        } else if (ww == w) {
-            gsum = sum;
+        unsigned incl = gsum;
 #pragma unroll
-            for (int q = 0; q < DPT; ++q) gh[q] = v[q];
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
         }
-    }
-    unsigned incl = gsum;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const unsigned u = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += u;
-    }
-    unsigned goff[DPT];
-    {
         unsigned run = before + incl - gsum;
 #pragma unroll
         for (int q = 0; q < DPT; ++q) {
@@ -206,38 +227,41 @@ __global__ __launch_bounds__(kSortBS) void k_onesweep(
     for (int r = 0; r < kRounds; ++r)
         if (seg + r * 64 + lane < n) atomicAdd(&s_base[w][(key[r] >> shift) & kDigitMask], 1u);
     __syncthreads();
-    // thread t owns digits t * DPT .. t * DPT + DPT - 1: publish all, then look back
-    unsigned cnt[DPT];
+    // thread t < 256 owns digit t: publish its count, then look back
+    if (owner) {
+        unsigned cnt[DPT];
 #pragma unroll
-    for (int q = 0; q < DPT; ++q) {
-        const int d = t * DPT + q;
-        cnt[q] = 0;
+        for (int q = 0; q < DPT; ++q) {
+            const int d = t * DPT + q;
+            cnt[q] = 0;
 #pragma unroll
-        for (int ww = 0; ww < kWaves; ++ww) cnt[q] += s_base[ww][d];
-        __hip_atomic_store(&lookback_p[tile * kBuckets + d],
-                           (tile == 0 ? kFlagInc : kFlagAgg) | cnt[q], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-    unsigned excl[DPT];
-#pragma unroll
-    for (int q = 0; q < DPT; ++q) {
-        const int d = t * DPT + q;
-        excl[q] = 0;
-        if (tile > 0) {
-            excl[q] = look_back(lookback_p + d, tile, kBuckets, stats, spin_limit);
-            __hip_atomic_store(&lookback_p[tile * kBuckets + d], kFlagInc | (excl[q] + cnt[q]),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int ww = 0; ww < kWaves; ++ww) cnt[q] += s_base[ww][d];
+            __hip_atomic_store(&lookback_p[tile * kBuckets + d],
+                               (tile == 0 ? kFlagInc : kFlagAgg) | cnt[q], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
-    }
+        unsigned excl[DPT];
 #pragma unroll
-    for (int q = 0; q < DPT; ++q) {
-        const int d = t * DPT + q;
-        unsigned run = goff[q] + excl[q];
+        for (int q = 0; q < DPT; ++q) {
+            const int d = t * DPT + q;
+            excl[q] = 0;
+            if (tile > 0) {
+                excl[q] = look_back(lookback_p + d, tile, kBuckets, stats, spin_limit);
+                __hip_atomic_store(&lookback_p[tile * kBuckets + d],
+                                   kFlagInc | (excl[q] + cnt[q]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
 #pragma unroll
-        for (int ww = 0; ww < kWaves; ++ww) {
-            const unsigned c = s_base[ww][d];
-            s_base[ww][d] = run;
-            run += c;
+        for (int q = 0; q < DPT; ++q) {
+            const int d = t * DPT + q;
+            unsigned run = goff[q] + excl[q];
+#pragma unroll
+            for (int ww = 0; ww < kWaves; ++ww) {
+                const unsigned c = s_base[ww][d];
+                s_base[ww][d] = run;
+                run += c;
+            }
         }
     }
     __syncthreads();
@@ -271,12 +295,23 @@ __global__ __launch_bounds__(kSortBS) void k_onesweep(
 // so the real limit is several times higher).
 constexpr int kTicketlessTiles = 256;
 
-inline void launch_onesweep(int items, int nblk, hipStream_t st, const uint64_t *kin,
+inline void launch_onesweep(int items, int bs, int nblk, hipStream_t st, const uint64_t *kin,
                             const int *iin, uint64_t *kout, int *iout, int n, int shift,
                             const unsigned *ghist_p, unsigned *lookback_p, int *ticket_p,
                             unsigned long long *stats)
 {
     const bool ticket = rg::g_sort_ticket || nblk > kTicketlessTiles;
+    if (bs == kSortBSWide) {  // 4 keys per thread
+        if (ticket)
+            k_onesweep<kSortItemsSmall, true, kSortBSWide><<<nblk, kSortBSWide, 0, st>>>(
+                kin, iin, kout, iout, n, shift, ghist_p, lookback_p, ticket_p, stats,
+                rg::g_spin_limit);
+        else
+            k_onesweep<kSortItemsSmall, false, kSortBSWide><<<nblk, kSortBSWide, 0, st>>>(
+                kin, iin, kout, iout, n, shift, ghist_p, lookback_p, ticket_p, stats,
+                rg::g_spin_limit);
+        return;
+    }
 #define RG_ONESWEEP(IT, TK)                                                                   \
     k_onesweep<IT, TK><<<nblk, kSortBS, 0, st>>>(kin, iin, kout, iout, n, shift, ghist_p,     \
                                                  lookback_p, ticket_p, stats, rg::g_spin_limit)
