@@ -37,8 +37,8 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK_GINST = 1024 * 2.4 / 2
 VALU64_PEAK_GINST = 1024 * 2.4 / 4
 # direct-sum kernel each precision launches at n = 262,144 (default shape)
-DIRECT_KERNELS = {"fp64": "k_direct<0, 1>", "fp32": "k_direct<1, 2>",
-                  "fp32c": "k_direct<2, 4>"}
+DIRECT_KERNELS = {"fp64": "k_direct<0, 1, false>", "fp32": "k_direct_sym<1, false, false>",
+                  "fp32c": "k_direct_sym<2, false, false>"}
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_INTERACTION = 20

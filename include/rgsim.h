@@ -13,9 +13,11 @@
  *  - Scratch memory: a BH handle owns its scratch.  rg_energy,
  *    rg_morton_codes, and the FP32/FP32C direct sums and rg_direct_vjp when
  *    they split j (from n = 257 up by default; at most 2^24 x 32 B) use a
- *    library buffer per (device, stream), held until the process ends.
- *    Calls on different streams never share it; calls issued to one stream
- *    from several host threads at once race.
+ *    library buffer per (device, stream), held until the process ends; the
+ *    symmetric FP32/FP32C sums ("direct_sym", from n = 131,072) take about
+ *    12 n^2 / 1024 + 24 n^2 / 4096 B there (1.0 GB at n = 262,144, at most
+ *    3 GiB).  Calls on different streams never share it; calls issued to
+ *    one stream from several host threads at once race.
  *  - `precision`: RG_FP64 reproduces the reference arithmetic; RG_FP32 is
  *    plain FP32 pair arithmetic; RG_FP32C is FP32 pair arithmetic on
  *    double-float (hi+lo) positions with compensated accumulation.
@@ -89,6 +91,13 @@ const char *rg_last_error(void);
  *                 slice order in FP64, so results are deterministic for a
  *                 given n and setting; 1 gives the unsplit sum.  FP64 direct
  *                 sums are never split (they keep the reference's j order).
+ *   "direct_sym"  the symmetric FP32/FP32C direct sums (each pair evaluated
+ *                 once, its term added to both particles; deterministic,
+ *                 the j-side sums added in FP64): 0 (default: from n =
+ *                 131,072 while "direct_split" and "direct_ipt" are 0 and the
+ *                 scratch stays within 3 GiB), 1 always, -1 never; 2 and 3
+ *                 as 0 and 1 with the j-side sums rotating through the lanes
+ *                 in FP32 (flushed to FP64 every 32 steps)
  *   "walk_block"  threads per block of the BH walk: 64, 128, 256 or 512
  *   "walk_mode"   FP32 BH walk: -1 auto (default: 7), 0 one lane per
  *                 particle, 1 two lanes per particle, 2 one wave walking

@@ -74,6 +74,7 @@ extern "C" int rg_set_param(const char *name, int64_t value)
     RG_REQUIRE(name, RG_ERR_ARG, "null name");
     if (std::strcmp(name, "direct_ipt") == 0) return rg::rg_direct_set_ipt_impl(value);
     if (std::strcmp(name, "direct_split") == 0) return rg::rg_direct_set_split_impl(value);
+    if (std::strcmp(name, "direct_sym") == 0) return rg::rg_direct_set_sym_impl(value);
     if (std::strcmp(name, "fp64_pipe") == 0) return rg::rg_direct_set_fp64_pipe_impl(value);
     if (std::strcmp(name, "sort_ticket") == 0) {
         RG_REQUIRE(value == 0 || value == 1, RG_ERR_ARG, "sort_ticket must be 0 or 1");

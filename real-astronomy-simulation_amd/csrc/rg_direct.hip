@@ -528,11 +528,406 @@ __global__ __launch_bounds__(256) void k_direct_reduce(const double *__restrict_
     }
 }
 
+// ---------------------------------------------- symmetric FP32/FP32C sums
+// Each unordered pair is evaluated once (Newton's third law): i-block B
+// (1024 particles, four per lane) against the j-tiles beyond it adds the
+// pair term to i (i-side, in registers, compensated as in k_direct) and
+// its opposite, with m_i for m_j, to j (j-side).  The j-side needs no
+// cross-lane reduction: in step s of a group of 64 j, lane l takes j = (l +
+// s) mod 64, so every j visits each lane once per group, and the lane adds
+// its four-term j-side partial to the wave's FP64 accumulator of that j in
+// LDS (a non-returning ds_add_f64; within one instruction the 64 lanes hold
+// 64 different j, and one wave's instructions reach LDS in order, so there
+// is no race and the order is fixed).  After a tile the four waves'
+// accumulators are added in wave order and written to jpart[B][axis][j -
+// 1024 (B + 1)].  A block's own four tiles (the diagonal) are summed
+// one-sided.  k_direct_sym_reduce adds, per particle, its i-side partials in
+// segment order and then the j-side partials of the earlier i-blocks in
+// block order, all in FP64: the result is deterministic for a given n.  A
+// pair costs about 23 VALU instructions (one rsqrt) against 19 in k_direct,
+// for half the pairs.
+constexpr int kSymIpt = 4;
+constexpr int kSymIB = kBS * kSymIpt;       // i-particles per block
+constexpr int kSymTilesIB = kSymIB / kBS;   // their tiles (4)
+constexpr int kSymSeg = 16;                 // j-tiles per segment of a block row
+
+// Segments of block row b: the diagonal, then ceil(beyond / kSymSeg).
+__host__ __device__ __forceinline__ int sym_nseg(int b, int nt)
+{
+    const int beyond = nt - kSymTilesIB * (b + 1);
+    return 1 + (beyond > 0 ? (beyond + kSymSeg - 1) / kSymSeg : 0);
+}
+// First j-side entry (doubles) of block row b: rows hold 3 x (n - 1024 (b + 1)).
+__host__ __device__ __forceinline__ int64_t sym_jrow(int b, int n)
+{
+    return 3 * (static_cast<int64_t>(b) * n - static_cast<int64_t>(kSymIB) / 2 * b * (b + 1));
+}
+
+// Lane l gets lane (l + 1) mod 64's value (DPP wave_rol:1).
+__device__ __forceinline__ float wave_rol1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF,
+                                                      false));
+}
+
+// JROT: the j-side sums rotate through the lanes in registers (FP32, one
+// DPP rotation per step fused into the add) instead of FP64 LDS adds.
+template <int MODE, bool HILO, bool JROT = false>
+__global__ __launch_bounds__(kBS, 4) void k_direct_sym(const double *__restrict__ pos,
+                                                    const float *__restrict__ pos_lo,
+                                                    const double *__restrict__ mass, int n,
+                                                    double G, double eps2,
+                                                    double *__restrict__ ipart,
+                                                    double *__restrict__ jpart)
+{
+    static_assert(MODE == 1 || MODE == 2, "FP32 and FP32C only");
+    constexpr int IPT = kSymIpt;
+    constexpr int kW = kBS / 64;
+    __shared__ float4 s_hi[kBS];
+    __shared__ float4 s_lo[MODE == 2 ? kBS : 1];
+    __shared__ double s_ja[kW][3][kBS];  // per wave: the j-side sums of the tile
+    const int B = blockIdx.y;
+    const int seg = blockIdx.x;
+    const int nt = (n + kBS - 1) / kBS;
+    const int nseg = sym_nseg(B, nt);
+    if (seg >= nseg) return;  // uniform: rows beyond the diagonal are shorter
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int base = B * kSymIB + tid;
+    float xh[IPT], yh[IPT], zh[IPT], xl[IPT], yl[IPT], zl[IPT], gmi[IPT];
+    float sx[IPT], sy[IPT], sz[IPT], cx[IPT], cy[IPT], cz[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int i = base + k * kBS;
+        const bool ok = i < n;
+        const double px = ok ? pos[3 * i] : 0.0;
+        const double py = ok ? pos[3 * i + 1] : 0.0;
+        const double pz = ok ? pos[3 * i + 2] : 0.0;
+        if constexpr (HILO) {
+            split_df_off(px, ok ? pos_lo[3 * i] : 0.f, xh[k], xl[k]);
+            split_df_off(py, ok ? pos_lo[3 * i + 1] : 0.f, yh[k], yl[k]);
+            split_df_off(pz, ok ? pos_lo[3 * i + 2] : 0.f, zh[k], zl[k]);
+        } else {
+            split_df(px, xh[k], xl[k]);
+            split_df(py, yh[k], yl[k]);
+            split_df(pz, zh[k], zl[k]);
+        }
+        if (MODE == 1) {  // plain FP32: the rounded position
+            xl[k] = yl[k] = zl[k] = 0.f;
+        }
+        gmi[k] = ok ? static_cast<float>(G * mass[i]) : 0.f;  // no j-side term from padding
+        sx[k] = sy[k] = sz[k] = 0.f;
+        cx[k] = cy[k] = cz[k] = 0.f;
+    }
+    const float eps2f = static_cast<float>(eps2);
+    const bool diag = seg == 0;
+    const int tb = kSymTilesIB * (B + 1);  // first tile beyond the block
+    const int tlo = diag ? kSymTilesIB * B : tb + (seg - 1) * kSymSeg;
+    const int thi = diag ? min(tb, nt) : min(nt, tb + seg * kSymSeg);
+    const int jrow0 = kSymIB * (B + 1);
+    double *jp = jpart + sym_jrow(B, n);
+    const int jlen = n - jrow0;
+
+    for (int t = tlo; t < thi; ++t) {
+        const int j = t * kBS + tid;
+        if (j < n) {
+            const double qx = pos[3 * j], qy = pos[3 * j + 1], qz = pos[3 * j + 2];
+            const float gm = static_cast<float>(G * mass[j]);
+            float hx, lx, hy, ly, hz, lz;
+            if constexpr (HILO) {
+                split_df_off(qx, pos_lo[3 * j], hx, lx);
+                split_df_off(qy, pos_lo[3 * j + 1], hy, ly);
+                split_df_off(qz, pos_lo[3 * j + 2], hz, lz);
+            } else {
+                split_df(qx, hx, lx);
+                split_df(qy, hy, ly);
+                split_df(qz, hz, lz);
+            }
+            s_hi[tid] = make_float4(hx, hy, hz, gm);
+            if constexpr (MODE == 2) s_lo[tid] = make_float4(lx, ly, lz, 0.f);
+        } else {
+            // past n: a massless point, so the i-side terms are zero (its
+            // j-side sums are never stored)
+            s_hi[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if constexpr (MODE == 2) s_lo[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (!diag) {
+#pragma unroll
+            for (int w = 0; w < kW; ++w) {
+                s_ja[w][0][tid] = 0.0;
+                s_ja[w][1][tid] = 0.0;
+                s_ja[w][2][tid] = 0.0;
+            }
+        }
+        __syncthreads();
+        const int cnt = min(kBS, n - t * kBS);
+        if (diag) {  // uniform: one-sided, as k_direct
+            for (int c0 = 0; c0 < cnt; c0 += kChunk) {
+                const int c1 = min(c0 + kChunk, cnt);
+                float px[IPT], py[IPT], pz[IPT];
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
+#pragma unroll 4
+                for (int jj = c0; jj < c1; ++jj) {
+                    const float4 q = s_hi[jj];
+                    float4 l = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if constexpr (MODE == 2) l = s_lo[jj];
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) {
+                        const float dx = MODE == 2 ? (q.x - xh[k]) + (l.x - xl[k]) : q.x - xh[k];
+                        const float dy = MODE == 2 ? (q.y - yh[k]) + (l.y - yl[k]) : q.y - yh[k];
+                        const float dz = MODE == 2 ? (q.z - zh[k]) + (l.z - zl[k]) : q.z - zh[k];
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        const float sc = q.w * (inv * inv) * inv;
+                        px[k] = fmaf(sc, dx, px[k]);
+                        py[k] = fmaf(sc, dy, py[k]);
+                        pz[k] = fmaf(sc, dz, pz[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) {
+                    float e;
+                    two_sum(sx[k], px[k], sx[k], e);
+                    cx[k] += e;
+                    two_sum(sy[k], py[k], sy[k], e);
+                    cy[k] += e;
+                    two_sum(sz[k], pz[k], sz[k], e);
+                    cz[k] += e;
+                }
+            }
+        } else {
+            // groups of 64 j; chunks of 32 steps for the i-side compensation
+            for (int g0 = 0; g0 < cnt; g0 += 64) {
+                // JROT: R* hold, at step st, lane l's running sum for j = (l + st)
+                // mod 64 of the group: R = rol(R) + this step's term, flushed to
+                // FP64 at the end of each 32-step chunk
+                float Rx = 0.f, Ry = 0.f, Rz = 0.f;
+                for (int c0 = 0; c0 < 64; c0 += kChunk) {
+                    float px[IPT], py[IPT], pz[IPT];
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
+#pragma unroll 2
+                    for (int st = c0; st < c0 + kChunk; ++st) {
+                        const int jl = g0 + ((lane + st) & 63);
+                        const float4 q = s_hi[jl];
+                        float4 l = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if constexpr (MODE == 2) l = s_lo[jl];
+                        float ajx = 0.f, ajy = 0.f, ajz = 0.f;
+#pragma unroll
+                        for (int k = 0; k < IPT; ++k) {
+                            const float dx =
+                                MODE == 2 ? (q.x - xh[k]) + (l.x - xl[k]) : q.x - xh[k];
+                            const float dy =
+                                MODE == 2 ? (q.y - yh[k]) + (l.y - yl[k]) : q.y - yh[k];
+                            const float dz =
+                                MODE == 2 ? (q.z - zh[k]) + (l.z - zl[k]) : q.z - zh[k];
+                            const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2f)));
+                            const float inv = __builtin_amdgcn_rsqf(r2);
+                            const float inv3 = (inv * inv) * inv;
+                            const float si = q.w * inv3;
+                            const float sj = gmi[k] * inv3;
+                            px[k] = fmaf(si, dx, px[k]);
+                            py[k] = fmaf(si, dy, py[k]);
+                            pz[k] = fmaf(si, dz, pz[k]);
+                            ajx = fmaf(sj, dx, ajx);
+                            ajy = fmaf(sj, dy, ajy);
+                            ajz = fmaf(sj, dz, ajz);
+                        }
+                        if constexpr (JROT) {  // at st = 0 the rotated sums are zeros
+                            Rx = wave_rol1(Rx) + ajx;
+                            Ry = wave_rol1(Ry) + ajy;
+                            Rz = wave_rol1(Rz) + ajz;
+                        } else {
+                            __hip_atomic_fetch_add(&s_ja[wave][0][jl], static_cast<double>(ajx),
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+                            __hip_atomic_fetch_add(&s_ja[wave][1][jl], static_cast<double>(ajy),
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+                            __hip_atomic_fetch_add(&s_ja[wave][2][jl], static_cast<double>(ajz),
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) {
+                        float e;
+                        two_sum(sx[k], px[k], sx[k], e);
+                        cx[k] += e;
+                        two_sum(sy[k], py[k], sy[k], e);
+                        cy[k] += e;
+                        two_sum(sz[k], pz[k], sz[k], e);
+                        cz[k] += e;
+                    }
+                    if constexpr (JROT) {
+                        // end of the chunk: lane l's sum (32 terms at most) belongs
+                        // to j = (l + c0 + 31) mod 64; add it to that j's FP64
+                        // accumulator and start again from zero
+                        const int jf = g0 + ((lane + c0 + kChunk - 1) & 63);
+                        __hip_atomic_fetch_add(&s_ja[wave][0][jf], static_cast<double>(Rx),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        __hip_atomic_fetch_add(&s_ja[wave][1][jf], static_cast<double>(Ry),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        __hip_atomic_fetch_add(&s_ja[wave][2][jf], static_cast<double>(Rz),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        Rx = Ry = Rz = 0.f;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (!diag && j < n) {  // j-side of j = t * 256 + tid: the waves' sums in wave order
+            double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+            for (int w = 0; w < kW; ++w) {
+                ax += s_ja[w][0][tid];
+                ay += s_ja[w][1][tid];
+                az += s_ja[w][2][tid];
+            }
+            const int jj = j - jrow0;
+            jp[jj] = -ax;  // the force on j is minus the sum of sj d
+            jp[jlen + jj] = -ay;
+            jp[2 * static_cast<int64_t>(jlen) + jj] = -az;
+        }
+        __syncthreads();  // s_hi, s_lo and s_ja are rewritten by the next tile
+    }
+    // i-side partial of this segment
+    int64_t segoff = 0;
+    for (int b = 0; b < B; ++b) segoff += sym_nseg(b, nt);
+    double *ip = ipart + (segoff + seg) * 3 * kSymIB;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int li = tid + k * kBS;
+        ip[li] = static_cast<double>(sx[k]) + static_cast<double>(cx[k]);
+        ip[kSymIB + li] = static_cast<double>(sy[k]) + static_cast<double>(cy[k]);
+        ip[2 * kSymIB + li] = static_cast<double>(sz[k]) + static_cast<double>(cz[k]);
+    }
+}
+
+// Per particle: its block row's i-side partials in segment order, then the
+// j-side partials of every earlier block row in row order (FP64), then the
+// epilogue.
+__global__ __launch_bounds__(256) void k_direct_sym_reduce(const double *__restrict__ ipart,
+                                                           const double *__restrict__ jpart,
+                                                           int n, const double *__restrict__ pos,
+                                                           double *__restrict__ acc,
+                                                           double *__restrict__ vel, double G,
+                                                           double dt_half, Epilogue epi)
+{
+    const int nt = (n + kBS - 1) / kBS;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int B = i / kSymIB;
+        const int li = i - B * kSymIB;
+        int64_t segoff = 0;
+        for (int b = 0; b < B; ++b) segoff += sym_nseg(b, nt);
+        const int ns = sym_nseg(B, nt);
+        const double *ip = ipart + segoff * 3 * kSymIB;
+        double ax = ip[li], ay = ip[kSymIB + li], az = ip[2 * kSymIB + li];
+        for (int s = 1; s < ns; ++s) {
+            const double *q = ip + static_cast<int64_t>(s) * 3 * kSymIB;
+            ax = ax + q[li];
+            ay = ay + q[kSymIB + li];
+            az = az + q[2 * kSymIB + li];
+        }
+        for (int b = 0; b < B; ++b) {
+            const int64_t len = n - static_cast<int64_t>(kSymIB) * (b + 1);
+            const double *q = jpart + sym_jrow(b, n) + (i - kSymIB * (b + 1));
+            ax = ax + q[0];
+            ay = ay + q[len];
+            az = az + q[2 * len];
+        }
+        finish_particle(i, ax, ay, az, pos, acc, vel, G, dt_half, epi);
+    }
+}
+
+// Scratch (doubles) of the symmetric sum: the i-side partials of every
+// segment, then the j-side rows.
+int64_t sym_scratch_doubles(int n)
+{
+    const int nt = (n + kBS - 1) / kBS;
+    const int nb = (n + kSymIB - 1) / kSymIB;
+    int64_t segs = 0;
+    for (int b = 0; b < nb; ++b) segs += sym_nseg(b, nt);
+    return segs * 3 * kSymIB + sym_jrow(nb > 0 ? nb - 1 : 0, n);
+}
+
+// The symmetric sum is used for FP32/FP32C from n = kSymFrom (four
+// i-particles per lane there anyway) while its scratch fits kSymMaxBytes;
+// "direct_sym" forces it on (1) or off (-1), and any explicit "direct_split"
+// or "direct_ipt" selects the j-split kernel.
+constexpr int kSymFrom = 131072;
+constexpr int64_t kSymMaxBytes = int64_t(3) << 30;
+int g_direct_sym = 0;
+int g_sym_jrot = 0;  // j-side sums: 0 FP64 LDS adds, 1 rotating FP32 registers
+
+bool sym_wanted(int mode, int n)
+{
+    if (mode != 1 && mode != 2) return false;
+    if (g_direct_sym < 0) return false;
+    if (g_direct_sym == 0 && (n < kSymFrom || g_direct_split || g_direct_ipt)) return false;
+    return n >= 2 && 8 * sym_scratch_doubles(n) <= kSymMaxBytes;
+}
+
+template <int MODE>
+int launch_direct_sym(const double *pos, const float *pos_lo, const double *mass, double *acc,
+                      double *vel, int n, double G, double eps2, double dt_half,
+                      const Epilogue &epi, hipStream_t st, const ScratchSrc &ss)
+{
+    const size_t need = 8 * static_cast<size_t>(sym_scratch_doubles(n));
+    const int nt = (n + kBS - 1) / kBS;
+    const int nb = (n + kSymIB - 1) / kSymIB;
+    const int maxseg = sym_nseg(0, nt);
+    const int64_t segs = sym_scratch_doubles(n) >= 0 ? [&] {
+        int64_t c = 0;
+        for (int b = 0; b < nb; ++b) c += sym_nseg(b, nt);
+        return c;
+    }() : 0;
+    void *buf = nullptr;
+    if (ss.ptr) {
+        RG_REQUIRE(need <= ss.cap, RG_ERR_CAPACITY,
+                   "n=%d: the symmetric sum needs %zu MiB of partials; the static scratch "
+                   "holds %zu MiB", n, (need + (1u << 20) - 1) >> 20, ss.cap >> 20);
+        buf = ss.ptr;
+    } else {
+        const int rc = rg::device_scratch(need, st, &buf);
+        if (rc != RG_OK) return rc;
+    }
+    double *ipart = static_cast<double *>(buf);
+    double *jpart = ipart + segs * 3 * kSymIB;
+    const dim3 grid(maxseg, nb);
+    if (g_sym_jrot) {
+        if (MODE == 2 && pos_lo)
+            k_direct_sym<MODE, true, true><<<grid, kBS, 0, st>>>(pos, pos_lo, mass, n, G, eps2,
+                                                                 ipart, jpart);
+        else
+            k_direct_sym<MODE, false, true><<<grid, kBS, 0, st>>>(pos, nullptr, mass, n, G, eps2,
+                                                                  ipart, jpart);
+    } else if (MODE == 2 && pos_lo) {
+        k_direct_sym<MODE, true><<<grid, kBS, 0, st>>>(pos, pos_lo, mass, n, G, eps2, ipart,
+                                                       jpart);
+    } else {
+        k_direct_sym<MODE, false><<<grid, kBS, 0, st>>>(pos, nullptr, mass, n, G, eps2, ipart,
+                                                        jpart);
+    }
+    RG_CHECK_LAUNCH();
+    k_direct_sym_reduce<<<elementwise_grid(n), 256, 0, st>>>(ipart, jpart, n, pos, acc, vel, G,
+                                                             dt_half, epi);
+    RG_CHECK_LAUNCH();
+    return RG_OK;
+}
+
 template <int MODE>
 int launch_direct(const double *pos, const float *pos_lo, const double *mass, double *acc,
                   double *vel, int n, double G, double eps2, double dt_half,
                   const Epilogue &epi, hipStream_t st, const ScratchSrc &ss)
 {
+    if constexpr (MODE == 1 || MODE == 2) {
+        if (sym_wanted(MODE, n))
+            return launch_direct_sym<MODE>(pos, pos_lo, mass, acc, vel, n, G, eps2, dt_half, epi,
+                                           st, ss);
+    }
     int ipt = g_direct_ipt;
     if (ipt == 0) {
         ipt = 1;
@@ -919,6 +1314,15 @@ int rg::rg_direct_set_split_impl(int64_t v)
 {
     RG_REQUIRE(v >= 0 && v <= kMaxSplit, RG_ERR_ARG, "direct_split must be in [0, %d]", kMaxSplit);
     g_direct_split = static_cast<int>(v);
+    return RG_OK;
+}
+
+int rg::rg_direct_set_sym_impl(int64_t v)
+{
+    RG_REQUIRE(v >= -1 && v <= 3, RG_ERR_ARG, "direct_sym must be -1, 0, 1, 2 or 3");
+    // 2 and 3: as 0 and 1 with the j-side sums rotating through registers
+    g_direct_sym = v == 2 ? 0 : (v == 3 ? 1 : static_cast<int>(v));
+    g_sym_jrot = v >= 2 ? 1 : 0;
     return RG_OK;
 }
 

@@ -45,6 +45,7 @@ constexpr int kDefaultSpinLimit = 1 << 20;
 // Tunables behind rg_set_param.
 int rg_direct_set_ipt_impl(int64_t v);
 int rg_direct_set_split_impl(int64_t v);
+int rg_direct_set_sym_impl(int64_t v);
 int rg_direct_set_fp64_pipe_impl(int64_t v);
 int rg_direct_set_fp32c_d64_impl(int64_t v);
 int rg_bh_set_walk_block_impl(int64_t v);
