@@ -79,6 +79,15 @@ const char *rg_last_error(void);
  *                 created afterwards: 0 (default: 1024 from 200,000 keys,
  *                 else 256), 256, or 1024 (with 4 keys per thread, whatever
  *                 "sort_items" says).  Same results either way
+ *   "sort_sample" the BH tree's sort: -1 (default) the sample sort from the
+ *                 handle's second compute when 4,096 <= n <= 1,572,864, 0
+ *                 always the 8-pass (4 with 30-bit keys) LSD radix sort, 1
+ *                 the sample sort from the second compute at any n up to
+ *                 1,572,864.  The sample sort takes splitters from the
+ *                 previous compute's order, moves the keys into up to 256
+ *                 buckets in one radix pass and sorts each bucket in one
+ *                 workgroup (in LDS up to 8,192 keys, else in global
+ *                 memory, counted in rg_bh_stats).  Same order either way
  *   "sort_items"  keys per thread in a 256-thread radix-sort tile, for BH and
  *                 SPH-grid handles created afterwards: 0 (default: 4 below
  *                 524,288 keys, 8 below 786,432, else 16), 4, 8 or 16.  Same
@@ -322,6 +331,9 @@ typedef struct {
     int64_t nonfinite;      /* non-finite input coordinates seen */
     int64_t walk_mode;      /* FP32 walk variant used (0..8, see "walk_mode"); -1 FP64 */
     int64_t walk_waves;     /* waves per group of 64 particles in it (0: one lane each) */
+    int64_t sort_passes;    /* global radix passes of its sort: 8 or 4 (LSD), 1 (sample sort) */
+    int64_t sort_buckets;   /* sample sort: buckets (0 for the LSD sort) */
+    int64_t sort_spilled;   /* sample sort: buckets too large for LDS, sorted in global memory */
 } rg_bh_stats_t;
 
 /* Statistics of the last compute.  Synchronises that compute's stream.

@@ -28,6 +28,7 @@
 //
 // Node ids follow the reference: leaves 0..n-1 in sorted order, internal
 // node i has id i+n, root id n.
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -103,7 +104,7 @@ __device__ __forceinline__ void block_box_reduce(double v[7], double (*s_w)[7])
 // walk, [1] the most one particle visited, [2] non-finite coordinates,
 // [3] look-back waits that gave up (radix sort, treelet ranking: the order
 // and tree are then invalid), [4] walks or preorder chains that ran out of
-// bounds (a corrupt tree).
+// bounds (a corrupt tree), [6] sample-sort buckets sorted in global memory.
 constexpr int kStatWords = 8;
 
 // A look-back wait of this compute gave up (stats[3]): every kernel after
@@ -132,6 +133,7 @@ __device__ __forceinline__ void reset_counters(unsigned long long *__restrict__ 
         stats[1] = 0ull;
         stats[3] = 0ull;
         stats[4] = 0ull;
+        stats[6] = 0ull;
     }
 }
 
@@ -299,6 +301,107 @@ __global__ __launch_bounds__(kBoxBS) void k_morton(const double *__restrict__ po
         lookback[k] = 0u;
 }
 
+// Sample-sort form of k_morton: keys, and each key's bucket among the
+// nbuckets key ranges that nbuckets - 1 splitters cut, with the histogram of
+// bucket ids (the one digit the bucket pass sorts by).  The splitters are
+// the composites (key, index) at every stride-th position of the previous
+// compute's sorted keys and order: key values are places in the box, and
+// the count of particles below a place changes only by the particles that
+// cross it, so each bucket stays near stride keys while particles move
+// little between computes (the current keys of sampled particles would not:
+// one that crosses a coarse octree boundary jumps far in the order, and its
+// neighbours' buckets merge).  A bucket's share only affects speed, never
+// the order: k_bucket_sort sorts a bucket of any size, and every block
+// sorts the splitters in LDS first, so any values work.  Bucket of particle
+// i: the number of splitters <= (key_i, i).
+__device__ __forceinline__ uint64_t morton_key_at(const double *__restrict__ pos, int i,
+                                                  const double lo[3], double scale, double qmax)
+{
+    uint64_t key = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double t = (pos[3 * i + a] - lo[a]) * scale;
+        if (!(t >= 0.0)) t = 0.0;
+        if (t > qmax) t = qmax;
+        key |= part1by2(static_cast<uint64_t>(t)) << a;
+    }
+    return key;
+}
+
+__global__ __launch_bounds__(kBoxBS) void k_morton_sample(
+    const double *__restrict__ pos, int n, int bits, const double *__restrict__ part, int nb,
+    double *__restrict__ box, unsigned long long *__restrict__ stats, uint64_t *__restrict__ keys,
+    const uint64_t *__restrict__ prev_keys, const int *__restrict__ prev_idx, int stride,
+    int nbuckets, uint8_t *__restrict__ bkt,
+    unsigned *__restrict__ ghist, unsigned *__restrict__ lookback, int lb_words,
+    int *__restrict__ tl_of)
+{
+    static_assert(kBoxBS == kBuckets, "one thread per splitter slot and per bucket");
+    __shared__ unsigned s_h[kBuckets];
+    __shared__ uint64_t s_sk[kBuckets];
+    __shared__ int s_sp[kBuckets];
+    __shared__ double s_w[kBoxBS / 64][7];
+    const int t = threadIdx.x;
+    s_h[t] = 0u;
+    double lo[3], scale;
+    const double bad = box_from_partials(part, nb, bits, lo, &scale, s_w);  // has a block barrier
+    if (blockIdx.x == 0 && t == 0) {
+        box[0] = lo[0];
+        box[1] = lo[1];
+        box[2] = lo[2];
+        box[3] = scale;
+        if (stats) stats[2] = static_cast<unsigned long long>(bad);
+    }
+    const double qmax = static_cast<double>((1ull << bits) - 1ull);
+    // splitter t (t < nbuckets - 1); the slots past them sort last
+    uint64_t sk = ~0ull;
+    int sp = 0x7fffffff;
+    if (t < nbuckets - 1) {
+        sk = prev_keys[(t + 1) * stride];
+        sp = prev_idx[(t + 1) * stride];
+    }
+    s_sk[t] = sk;
+    s_sp[t] = sp;
+    __syncthreads();
+    // bitonic sort of the 256 composites, ascending
+    for (int k = 2; k <= kBuckets; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int u = t ^ j;
+            if (u > t) {
+                const uint64_t ak = s_sk[t], bk = s_sk[u];
+                const int ap = s_sp[t], bp = s_sp[u];
+                const bool b_less = bk < ak || (bk == ak && bp < ap);
+                const bool up = (t & k) == 0;
+                if (b_less == up) {
+                    s_sk[t] = bk;
+                    s_sk[u] = ak;
+                    s_sp[t] = bp;
+                    s_sp[u] = ap;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = blockIdx.x * kBoxBS + t; i < n; i += gridDim.x * kBoxBS) {
+        const uint64_t key = morton_key_at(pos, i, lo, scale, qmax);
+        keys[i] = key;
+        if (tl_of) tl_of[i] = -1;  // set again by the build for the first leaf of each treelet
+        // splitters <= (key, i): at most nbuckets - 1 <= 255 of the 256 slots
+        int c = 0;
+#pragma unroll
+        for (int step = kBuckets / 2; step > 0; step >>= 1) {
+            const uint64_t k2 = s_sk[c + step - 1];
+            if (k2 < key || (k2 == key && s_sp[c + step - 1] <= i)) c += step;
+        }
+        bkt[i] = static_cast<uint8_t>(c);
+        atomicAdd(&s_h[c], 1u);
+    }
+    __syncthreads();
+    if (t < nbuckets && s_h[t]) atomicAdd(&ghist[t], s_h[t]);
+    for (int k = blockIdx.x * kBoxBS + t; k < lb_words; k += gridDim.x * kBoxBS)
+        lookback[k] = 0u;
+}
+
 // The opening kick + drift fused into the box pass (rg_bh_step).
 struct KickDrift {
     double *pos;
@@ -307,10 +410,20 @@ struct KickDrift {
     double dt_half, dt;
 };
 
+// Splitters and bucket ids for the sample-sort form of the key pass.
+struct SampleSort {
+    const uint64_t *prev_keys;  // the previous compute's sorted keys
+    const int *prev_idx;        // and order (not the buffers this pass writes)
+    int stride;                 // splitter spacing in them
+    int nbuckets;               // <= kMaxBuckets
+    uint8_t *bkt;               // bucket id per particle (out)
+};
+
 int launch_bbox_morton(const double *pos, int n, int bits, double *part, double *box,
                        unsigned long long *stats, uint64_t *keys, int passes, unsigned *ghist,
                        int *tickets, unsigned *lookback, int lb_words, int *counts, int *tl_of,
-                       hipStream_t st, const KickDrift *kd = nullptr)
+                       hipStream_t st, const KickDrift *kd = nullptr,
+                       const SampleSort *ss = nullptr)
 {
     // at most 1024 partials (the part buffer's size) either way
     int nb;
@@ -325,8 +438,14 @@ int launch_bbox_morton(const double *pos, int n, int bits, double *part, double 
     RG_CHECK_LAUNCH();
     // each block flushes passes x kBuckets global atomics: keep the grid modest
     const int mg = min((n + kBoxBS - 1) / kBoxBS, passes > 0 ? 256 : 2048);
-    k_morton<<<mg, kBoxBS, 0, st>>>(pos, n, bits, part, nb, box, stats, keys, passes, ghist,
-                                    lookback, lb_words, tl_of);
+    if (ss)
+        k_morton_sample<<<mg, kBoxBS, 0, st>>>(pos, n, bits, part, nb, box, stats, keys,
+                                               ss->prev_keys, ss->prev_idx, ss->stride,
+                                               ss->nbuckets, ss->bkt,
+                                               ghist, lookback, lb_words, tl_of);
+    else
+        k_morton<<<mg, kBoxBS, 0, st>>>(pos, n, bits, part, nb, box, stats, keys, passes, ghist,
+                                        lookback, lb_words, tl_of);
     RG_CHECK_LAUNCH();
     return RG_OK;
 }
@@ -1964,9 +2083,12 @@ struct rg_bh {
     bool computed = false;
     int walk_used = -1;   // walk variant of the last compute (-1 FP64)
     int walk_waves = 0;   // waves per group of 64 particles in that walk (0: per lane)
+    int sort_passes = 0;  // global radix passes of the last compute's sort
+    int sort_buckets = 0; // its sample-sort buckets (0: LSD sort)
 
     uint64_t *keys[2] = {nullptr, nullptr};
     int *idx[2] = {nullptr, nullptr};
+    uint8_t *bkt = nullptr;       // sample sort: bucket id per particle
     unsigned *ghist = nullptr;    // kMaxPasses x kBuckets global digit counts
     unsigned *lookback = nullptr; // passes x nblk x kBuckets look-back words
     int *tickets = nullptr;
@@ -2111,6 +2233,7 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->keys[1], 8 * N);
     ALLOC(h->idx[0], 4 * N);
     ALLOC(h->idx[1], 4 * N);
+    ALLOC(h->bkt, N);
     ALLOC(h->ghist, 4 * kBuckets * kMaxPasses);
     ALLOC(h->tickets, 4 * kMaxPasses);
     ALLOC(h->lookback, 4 * kBuckets * static_cast<size_t>(h->nblk) * h->passes);
@@ -2170,6 +2293,26 @@ int bh_take_abort(rg_bh *h)
     return RG_OK;
 }
 
+// The sample sort needs the previous compute's order for its splitters and
+// at most kMaxBuckets buckets of about kSampleTarget keys (LDS holds
+// kBucketCap; buckets stay near their previous share while particles move
+// little between computes).
+constexpr int kSampleTarget = 1024;
+constexpr int64_t kSampleMinN = 4096;
+constexpr int64_t kSampleMaxN = 1572864;  // 256 buckets of 6,144 keys
+
+int sample_buckets(int n)
+{
+    return static_cast<int>(std::min<int64_t>(kMaxBuckets,
+                                              std::max<int64_t>(1, n / kSampleTarget)));
+}
+
+bool sample_sort_wanted(const rg_bh *h)
+{
+    if (rg::g_sort_sample == 0 || !h->computed || h->n > kSampleMaxN) return false;
+    return rg::g_sort_sample == 1 || h->n >= kSampleMinN;
+}
+
 int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &out,
                void *stream, const KickDrift *kd = nullptr)
 {
@@ -2197,22 +2340,48 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     double *box = h->box;
     int rc = bh_mark(h, 0, st);
     if (rc != RG_OK) return rc;
-    const int lb_words = kBuckets * h->nblk * h->passes;
-    rc = launch_bbox_morton(pos, n, h->bits, part, box, h->stats, h->keys[0], h->passes,
-                            h->ghist, h->tickets, h->lookback, lb_words, h->counts, h->tl_of,
-                            st, kd);
-    if (rc != RG_OK) return rc;
-
-    // Stable LSD radix sort, kDigitBits-bit digits, one kernel per pass.
     int cur = 0;
-    for (int p = 0; p < h->passes; ++p) {
-        launch_onesweep(h->sort_items, h->sort_bs, h->nblk, st, h->keys[cur], p == 0 ? nullptr : h->idx[cur],
-                        h->keys[cur ^ 1], h->idx[cur ^ 1], n, kDigitBits * p,
-                        h->ghist + kBuckets * p,
-                        h->lookback + static_cast<size_t>(kBuckets) * h->nblk * p,
-                        h->tickets + p, h->stats);
+    if (sample_sort_wanted(h)) {
+        // Sample sort: buckets from the previous order's splitters, one
+        // stable pass over the bucket ids, then one workgroup per bucket.
+        // The splitters come from buffer f (the previous result); this
+        // compute's keys go to the other one, and the result back to f.
+        const int f = h->final_buf;
+        const int nbk = sample_buckets(n);
+        SampleSort ss{h->keys[f], h->idx[f], (n + nbk - 1) / nbk, nbk, h->bkt};
+        const int lb_words = kBuckets * h->nblk;
+        rc = launch_bbox_morton(pos, n, h->bits, part, box, h->stats, h->keys[f ^ 1], h->passes,
+                                h->ghist, h->tickets, h->lookback, lb_words, h->counts,
+                                h->tl_of, st, kd, &ss);
+        if (rc != RG_OK) return rc;
+        launch_onesweep(h->sort_items, h->sort_bs, h->nblk, st, h->keys[f ^ 1], nullptr,
+                        h->keys[f], h->idx[f], n, 0, h->ghist, h->lookback, h->tickets,
+                        h->stats, h->bkt);
         RG_CHECK_LAUNCH();
-        cur ^= 1;
+        k_bucket_sort<<<nbk, kBucketBS, 0, st>>>(h->keys[f], h->idx[f], h->keys[f ^ 1],
+                                                 h->idx[f ^ 1], h->ghist, 3 * h->bits, h->stats);
+        RG_CHECK_LAUNCH();
+        cur = f;
+        h->sort_passes = 1;
+        h->sort_buckets = nbk;
+    } else {
+        const int lb_words = kBuckets * h->nblk * h->passes;
+        rc = launch_bbox_morton(pos, n, h->bits, part, box, h->stats, h->keys[0], h->passes,
+                                h->ghist, h->tickets, h->lookback, lb_words, h->counts,
+                                h->tl_of, st, kd);
+        if (rc != RG_OK) return rc;
+        // Stable LSD radix sort, kDigitBits-bit digits, one kernel per pass.
+        for (int p = 0; p < h->passes; ++p) {
+            launch_onesweep(h->sort_items, h->sort_bs, h->nblk, st, h->keys[cur],
+                            p == 0 ? nullptr : h->idx[cur], h->keys[cur ^ 1], h->idx[cur ^ 1], n,
+                            kDigitBits * p, h->ghist + kBuckets * p,
+                            h->lookback + static_cast<size_t>(kBuckets) * h->nblk * p,
+                            h->tickets + p, h->stats);
+            RG_CHECK_LAUNCH();
+            cur ^= 1;
+        }
+        h->sort_passes = h->passes;
+        h->sort_buckets = 0;
     }
     h->final_buf = cur;
     rc = bh_mark(h, 1, st);
@@ -2461,7 +2630,7 @@ extern "C" int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out)
     RG_REQUIRE(h && out, RG_ERR_ARG, "null argument");
     RG_REQUIRE(h->computed, RG_ERR_ARG, "rg_bh_compute has not run on this handle");
     RG_HIP(hipStreamSynchronize(h->last_stream));
-    unsigned long long s[5] = {0, 0, 0, 0, 0};
+    unsigned long long s[7] = {0, 0, 0, 0, 0, 0, 0};
     RG_HIP(hipMemcpy(s, h->stats, sizeof(s), hipMemcpyDeviceToHost));
     out->n = h->n;
     out->visits_total = static_cast<int64_t>(s[0]);
@@ -2469,6 +2638,9 @@ extern "C" int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out)
     out->nonfinite = static_cast<int64_t>(s[2]);
     out->walk_mode = h->walk_used;
     out->walk_waves = h->walk_waves;
+    out->sort_passes = h->sort_passes;
+    out->sort_buckets = h->sort_buckets;
+    out->sort_spilled = static_cast<int64_t>(s[6]);
     if (s[3] != 0) {
         __atomic_store_n(h->host_flag, 0ull, __ATOMIC_RELEASE);  // reported here
         rg::set_error("BH radix sort: %llu look-back waits gave up (sort_spin_limit); the last "

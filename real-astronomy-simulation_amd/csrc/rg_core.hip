@@ -25,6 +25,7 @@ namespace rg {
 int g_sort_ticket = 0;
 int g_sort_items = 0;
 int g_sort_block = 0;
+int g_sort_sample = -1;
 int g_spin_limit = kDefaultSpinLimit;
 
 void set_error(const char *fmt, ...)
@@ -91,6 +92,12 @@ extern "C" int rg_set_param(const char *name, int64_t value)
         RG_REQUIRE(value == 0 || value == 256 || value == 1024, RG_ERR_ARG,
                    "sort_block must be 0, 256 or 1024");
         rg::g_sort_block = static_cast<int>(value);
+        return RG_OK;
+    }
+    if (std::strcmp(name, "sort_sample") == 0) {
+        RG_REQUIRE(value >= -1 && value <= 1, RG_ERR_ARG,
+                   "sort_sample must be -1 (auto), 0 (never) or 1 (whenever possible)");
+        rg::g_sort_sample = static_cast<int>(value);
         return RG_OK;
     }
     if (std::strcmp(name, "sort_spin_limit") == 0) {

@@ -37,6 +37,11 @@ extern int g_sort_items;
 // 256 or 1024 (with 4 keys per thread).
 extern int g_sort_block;
 
+// BH tree sort: -1 auto (the sample sort once a handle has an order from
+// its previous compute, for 4,096 <= n <= kSampleMaxN), 0 always the LSD
+// radix sort, 1 the sample sort whenever a previous order exists.
+extern int g_sort_sample;
+
 // Re-polls a decoupled look-back wait (radix sort, treelet ranking) makes
 // before it gives up and flags the failure; 0 gives up at once.
 extern int g_spin_limit;

@@ -31,7 +31,9 @@ _i = ctypes.c_int
 class BHStats(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("visits_total", ctypes.c_int64),
                 ("visits_max", ctypes.c_int64), ("nonfinite", ctypes.c_int64),
-                ("walk_mode", ctypes.c_int64), ("walk_waves", ctypes.c_int64)]
+                ("walk_mode", ctypes.c_int64), ("walk_waves", ctypes.c_int64),
+                ("sort_passes", ctypes.c_int64), ("sort_buckets", ctypes.c_int64),
+                ("sort_spilled", ctypes.c_int64)]
 
 
 class Potential(ctypes.Structure):

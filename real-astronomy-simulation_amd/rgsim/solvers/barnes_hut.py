@@ -140,7 +140,9 @@ class BarnesHut:
         s = _lib.BHStats()
         _lib.check(_lib.load().rg_bh_stats(self._h, ctypes.byref(s)), "rg_bh_stats")
         return {"n": s.n, "visits_total": s.visits_total, "visits_max": s.visits_max,
-                "nonfinite": s.nonfinite, "walk_mode": s.walk_mode, "walk_waves": s.walk_waves}
+                "nonfinite": s.nonfinite, "walk_mode": s.walk_mode, "walk_waves": s.walk_waves,
+                "sort_passes": s.sort_passes, "sort_buckets": s.sort_buckets,
+                "sort_spilled": s.sort_spilled}
 
     def set_timing(self, enable: bool) -> None:
         """Record HIP events around each phase of every compute."""

@@ -289,21 +289,27 @@ def test_bh_dropin_config_ic_and_after_200_steps(orc, bits):
     assert eng.step_count == 200
 
 
-def test_sort_lookback_giveup_is_reported(orc):
+@pytest.mark.parametrize("warm", [False, True])
+def test_sort_lookback_giveup_is_reported(orc, warm):
     """A radix-sort look-back wait that gives up is never silent.  With
     sort_spin_limit 0 every wait gives up at once (set from the host, so
     every kernel still runs to its end); the kernels after the sort then
     leave acc untouched, the next call on the handle raises with the
     radix-sort message, and so does stats().  With the default bound
-    restored the same handle computes the oracle's tree again."""
+    restored the same handle computes the oracle's tree again (by the
+    sample sort, from the aborted compute's invalid order as splitters).
+    warm: one good compute first, so the aborted one is a sample sort."""
     from rgsim import _lib
     from rgsim.solvers.barnes_hut import BarnesHut
-    print("bh sort give-up reporting, N=5000", flush=True)
+    print(f"bh sort give-up reporting, N=5000, warm={warm}", flush=True)
     pos, mass = CASES["diskbulge6000"]
     pos, mass = pos[:5000], mass[:5000]
     bh = BarnesHut(5000, theta=0.5, softening=0.05, G=1.0, precision="fp32")
     p, m = dev(pos), dev(mass)
     acc = torch.full_like(p, 7.0)
+    if warm:
+        bh.compute(p, m, torch.empty_like(p))
+        bh.stats()
     _lib.set_param("sort_spin_limit", 0)
     try:
         bh.compute(p, m, acc)  # asynchronous: returns before the kernels run
@@ -321,7 +327,7 @@ def test_sort_lookback_giveup_is_reported(orc):
     finally:
         _lib.set_param("sort_spin_limit", -1)
     bh.compute(p, m, acc)
-    bh.stats()
+    assert bh.stats()["sort_passes"] == 1  # the sample sort
     keys, order, left, right, parent = (t.cpu().numpy() for t in bh.export_tree())
     t = orc.bh_tree(pos, mass)
     assert np.array_equal(order, t["order"]) and np.array_equal(parent, t["parent"])
