@@ -363,23 +363,32 @@ __global__ __launch_bounds__(kBoxBS) void k_morton_sample(
     s_sk[t] = sk;
     s_sp[t] = sp;
     __syncthreads();
-    // bitonic sort of the 256 composites, ascending
-    for (int k = 2; k <= kBuckets; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const int u = t ^ j;
-            if (u > t) {
-                const uint64_t ak = s_sk[t], bk = s_sk[u];
-                const int ap = s_sp[t], bp = s_sp[u];
-                const bool b_less = bk < ak || (bk == ak && bp < ap);
-                const bool up = (t & k) == 0;
-                if (b_less == up) {
-                    s_sk[t] = bk;
-                    s_sk[u] = ak;
-                    s_sp[t] = bp;
-                    s_sp[u] = ap;
+    // Taken from a sorted result, the splitters are in order already; only
+    // an aborted previous compute leaves them out of order, and then the
+    // block sorts them (bitonic, ascending).
+    bool down = false;
+    if (t + 1 < kBuckets) {
+        const uint64_t nk = s_sk[t + 1];
+        down = nk < sk || (nk == sk && s_sp[t + 1] < sp);
+    }
+    if (__syncthreads_or(down)) {
+        for (int k = 2; k <= kBuckets; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const uint64_t ak = s_sk[t], bk = s_sk[u];
+                    const int ap = s_sp[t], bp = s_sp[u];
+                    const bool b_less = bk < ak || (bk == ak && bp < ap);
+                    const bool up = (t & k) == 0;
+                    if (b_less == up) {
+                        s_sk[t] = bk;
+                        s_sk[u] = ak;
+                        s_sp[t] = bp;
+                        s_sp[u] = ap;
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
     for (int i = blockIdx.x * kBoxBS + t; i < n; i += gridDim.x * kBoxBS) {
