@@ -1577,6 +1577,40 @@ __device__ __forceinline__ int xcd_run_block(int b, int nb, int R)
 constexpr int kChunkSide = 14;                     // chunk boundaries per side, at most
 constexpr int kChunkMaxB = 2 * kChunkSide + 2;    // + 1 inside the group + 1 end
 
+// Chunks of group g in k_walk32_chunked (its boundary count + 1).
+inline int group_chunks(long long g, long long n, long long dmin, int shift)
+{
+    const long long t0 = 64 * g;
+    const long long nact = std::min<long long>(64, n - t0);
+    const long long last = t0 + nact - 1;
+    int nr = 0, nl = 0;
+    for (long long d = dmin; last + d <= n - 1 && nr < kChunkSide; d <<= shift) ++nr;
+    for (long long d = dmin; t0 - d >= 0 && nl < kChunkSide; d <<= shift) ++nl;
+    return nr + nl + (nact > 33 ? 1 : 0) + 1;
+}
+
+// The most chunks any group has: the side counts change only where a
+// boundary distance d meets either end, so the maximum is taken at a group
+// next to one of those places (or at either end).
+inline int max_group_chunks(long long n, long long dmin, int shift)
+{
+    const long long ng = (n + 63) / 64;
+    int best = 0;
+    auto at = [&](long long g) {
+        if (g >= 0 && g < ng) best = std::max(best, group_chunks(g, n, dmin, shift));
+    };
+    at(0);
+    at(ng - 1);
+    int i = 0;
+    for (long long d = dmin; d <= n && i < kChunkSide; d <<= shift, ++i) {
+        for (long long q = -1; q <= 1; ++q) {
+            at(d / 64 + q);              // where t0 - d >= 0 starts
+            at((n - 1 - d) / 64 + q);    // where last + d <= n - 1 ends
+        }
+    }
+    return best;
+}
+
 // The wave walk of one group of 64 particles, cut into chunks of preorder
 // slots that the block's waves take in turn.  Most of a group's walk is in
 // its far field: the subtrees of leaves 2^10 .. 2^17 positions away in
@@ -2472,11 +2506,10 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
             const ChunkShape cs{g_chunk_dmin, shift, g_chunk_run};
             const int ng = (n + 63) / 64;
             h->walk_waves = waves;
-            // LDS for the chunk partial sums: the most chunks any group can
-            // have at this n (boundaries dmin * 2^(shift i) <= n - 1 per side)
-            int side = 0;
-            for (long long d = g_chunk_dmin; d <= n - 1 && side < kChunkSide; d <<= shift) ++side;
-            const int ncmax = 2 * side + 2;
+            // LDS for the chunk partial sums: the most chunks any group has
+            // at this n (a block more per CU fits than with the two-sided
+            // bound when no group can have full sides on both ends)
+            const int ncmax = max_group_chunks(n, g_chunk_dmin, shift);
             const size_t smem = sizeof(float) * 3 * 64 * static_cast<size_t>(ncmax);
             const float e2 = static_cast<float>(eps2);
             if (mode == 8 && waves == 8)
