@@ -117,17 +117,16 @@ def bench_bh(steps, warmup):
     t1 = time.perf_counter()
     wall = t1 - t0
     # phase times and the walk's launch duration from HIP events on the
-    # compute stream, over further steps; visits averaged over the same steps
+    # compute stream, over further steps (back to back: no synchronisation
+    # between them); visits of the last of those steps
     phase_steps = min(steps, 20)
     eng._bh.set_timing(True)
-    visits = 0
     for _ in range(phase_steps):
         eng.step(dt)
-        visits += eng._bh.stats()["visits_total"]
     tm = eng._bh.timing()
     eng._bh.set_timing(False)
     st_last = eng._bh.stats()
-    visits /= phase_steps
+    visits = st_last["visits_total"]
     walk_ms = tm["walk_ms"] / max(tm["count"], 1)
     walk_bytes = visits * WALK_RECORD_BYTES + n * WALK_PARTICLE_BYTES
     achieved = walk_bytes / (walk_ms * 1e-3) / 1e9
@@ -193,7 +192,10 @@ def bench_bh(steps, warmup):
                      "walk": walk_ms},
         "phase_hbm": phase_rooflines(n, {"sort": tm["sort_ms"] / max(tm["count"], 1),
                                          "build": tm["build_ms"] / max(tm["count"], 1),
-                                         "aggregate": tm["aggregate_ms"] / max(tm["count"], 1)}),
+                                         "aggregate": tm["aggregate_ms"] / max(tm["count"], 1)},
+                                     st_last["sort_passes"]),
+        "sort": {"global_passes": st_last["sort_passes"], "buckets": st_last["sort_buckets"],
+                 "buckets_in_global_memory": st_last["sort_spilled"]},
         "roofline": roofline,
         "walk_hbm": hbm,
         "energy": None,
@@ -249,12 +251,15 @@ def bench_bh30(state, dt=1e-4, steps=50):
 
 
 def phase_rooflines(n, phase_ms, passes=8):
-    """HBM rooflines of the tree-build phases from their algorithmic bytes:
-    keys + sort 24 (bbox read) + 24 + 8 (Morton) + passes x 24 (sort) B per
-    particle; build 24 B per internal node + 4 B preorder slot per node;
-    aggregation 2 x 80 B per node."""
+    """HBM rooflines of the tree-build phases from their algorithmic bytes.
+    Keys + sort per particle: 24 (bbox read) + 24 + 8 (Morton) + passes x 24
+    for the LSD radix sort; for the sample sort (passes == 1) 24 (bbox) +
+    24 + 8 + 1 (keys, bucket id) + 8 + 1 + 12 (bucket pass) + 24 (each bucket
+    read and written once).  Build 24 B per internal node + 4 B preorder slot
+    per node; aggregation 2 x 80 B per node."""
     m = 2 * n - 1
-    nbytes = {"sort": n * (24 + 24 + 8 + passes * 24),
+    sort_b = 24 + 24 + 8 + 1 + 8 + 1 + 12 + 24 if passes == 1 else 24 + 24 + 8 + passes * 24
+    nbytes = {"sort": n * sort_b,
               "build": 24 * (n - 1) + 4 * m,
               "aggregate": 160 * m}
     out = {}
@@ -609,7 +614,9 @@ def bench_bh_1m(steps=5):
     kd_gbs = 120 * n / (kd_ms * 1e-3) / 1e9
     out = {"n": n, "ms_per_step": 1e3 * wall / steps, "mps": n * steps / wall / 1e6,
            "phase_ms": dict(phases, walk=walk_ms),
-           "phase_hbm": phase_rooflines(n, phases),
+           "phase_hbm": phase_rooflines(n, phases, st["sort_passes"]),
+           "sort": {"global_passes": st["sort_passes"], "buckets": st["sort_buckets"],
+                    "buckets_in_global_memory": st["sort_spilled"]},
            "kick_drift": {"ms": kd_ms, "GB_per_s": kd_gbs, "frac_of_hbm_peak": kd_gbs / HBM_PEAK_GBS},
            "visits_per_particle": st["visits_total"] / n,
            "walk_effective_GBs": walk_bytes / (walk_ms * 1e-3) / 1e9,

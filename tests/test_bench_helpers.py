@@ -56,3 +56,16 @@ def test_issue_figures_from_profiles(bench):
                            6.0, 15509199223)
     assert b is not None and 19 < b["valu_instr_per_interaction"] < 22
     assert bench.kernel_issue("k_batch_force_512systems_sq.json", "no_such_kernel", 1.0, 1) is None
+
+
+def test_phase_rooflines_sort_bytes(bench):
+    """The sort phase's algorithmic bytes follow the path that ran: 248 B per
+    particle for the 8-pass LSD sort (bbox read, keys, 8 passes of keys and
+    indices read and written), 152 for 4 passes, 102 for the sample sort."""
+    ms = {"sort": 1.0, "build": 1.0, "aggregate": 1.0}
+    n = 1000
+    for passes, per in ((8, 248), (4, 152), (1, 102)):
+        r = bench.phase_rooflines(n, ms, passes)
+        assert r["sort"]["algorithmic_bytes"] == n * per
+        assert abs(r["sort"]["GB_per_s"] - n * per / 1e-3 / 1e9) < 1e-9
+    assert bench.phase_rooflines(n, ms)["sort"]["algorithmic_bytes"] == n * 248
