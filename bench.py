@@ -36,6 +36,10 @@ HBM_PEAK_GBS = 8000.0
 # SIMD every 2 cycles), an FP64 one 4 cycles (78.6 TFLOP/s)
 VALU_PEAK_GINST = 1024 * 2.4 / 2
 VALU64_PEAK_GINST = 1024 * 2.4 / 4
+# scalar instruction issue, G/s: each CU's scalar unit issues about one
+# instruction per cycle for all the waves of its 4 SIMDs (a probe of
+# independent s_add chains at full occupancy issued 581 G/s over the GPU)
+SALU_PEAK_GINST = 256 * 2.4
 # direct-sum kernel each precision launches at n = 262,144 (default shape)
 DIRECT_KERNELS = {"fp64": "k_direct<0, 1, false>", "fp32": "k_direct_sym<1, false, false>",
                   "fp32c": "k_direct_sym<2, false, false>"}
@@ -152,7 +156,8 @@ def bench_bh(steps, warmup):
             roofline_valu = {"bound": "valu", "achieved": rate, "peak": VALU_PEAK_GINST,
                              "unit": "G wave-instructions/s", "frac": rate / VALU_PEAK_GINST,
                              "kernel": kernel,
-                             "valu_instr_per_launch": rec["valu_instr_per_launch"]}
+                             "valu_instr_per_launch": rec["valu_instr_per_launch"],
+                             "salu_issue": salu_issue(rec, walk_ms)}
     # The walk against HBM: the bytes it must move at least (every FP32 node
     # record once, 24 B, and per particle its order index and FP64
     # acceleration, 28 B) and the bytes it did move past the L2 (rocprofv3
@@ -248,6 +253,19 @@ def bench_bh30(state, dt=1e-4, steps=50):
     w30 = time.perf_counter() - t30
     return {"morton_bits": 10, "steps": steps, "ms_per_step": 1e3 * w30 / steps,
             "mps": n * steps / w30 / 1e6}
+
+
+def salu_issue(rec, ms):
+    """Scalar-unit issue rate of a kernel: its scalar instructions per launch
+    (rocprofv3 SQ_INSTS_SALU in the committed summary `rec`) over the measured
+    launch time `ms`, against the CUs' scalar issue peak.  None when the
+    summary has no SQ_INSTS_SALU."""
+    instr = rec.get("per_launch", {}).get("SQ_INSTS_SALU")
+    if not instr:
+        return None
+    rate = instr / (ms * 1e-3) / 1e9
+    return {"achieved": rate, "peak": SALU_PEAK_GINST, "unit": "G instructions/s",
+            "frac": rate / SALU_PEAK_GINST, "salu_instr_per_launch": instr}
 
 
 def phase_rooflines(n, phase_ms, passes=8):

@@ -1,6 +1,6 @@
 """The bench's measurement helpers (CPU): the sum of term magnitudes behind
 the condition-normalised force error against a pairwise loop, and the VALU
-issue figures read from the committed rocprofv3 summaries."""
+and scalar issue figures read from the committed rocprofv3 summaries."""
 import importlib.util
 import os
 
@@ -56,6 +56,20 @@ def test_issue_figures_from_profiles(bench):
                            6.0, 15509199223)
     assert b is not None and 19 < b["valu_instr_per_interaction"] < 22
     assert bench.kernel_issue("k_batch_force_512systems_sq.json", "no_such_kernel", 1.0, 1) is None
+
+
+def test_walk_salu_issue_from_profile(bench):
+    """The walk's scalar-unit issue: SQ_INSTS_SALU per launch from the
+    committed summary over the launch time, against one scalar instruction
+    per cycle per CU (256 CUs at 2.4 GHz)."""
+    import json
+    with open(os.path.join(ROOT, "profiles", "k_walk32_chunked4_true_diskbulge120k_sq.json")) as f:
+        rec = json.load(f)
+    s = bench.salu_issue(rec, 0.3)
+    assert s["peak"] == pytest.approx(614.4)
+    assert s["achieved"] == pytest.approx(rec["per_launch"]["SQ_INSTS_SALU"] / 0.3e-3 / 1e9)
+    assert 0.3 < s["frac"] < 0.5
+    assert bench.salu_issue({"per_launch": {}}, 1.0) is None
 
 
 def test_phase_rooflines_sort_bytes(bench):
