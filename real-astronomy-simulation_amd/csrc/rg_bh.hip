@@ -1636,6 +1636,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     unsigned long long *__restrict__ stats, ChunkShape cs)
 {
     if (tree_aborted(stats)) return;
+    using v4f = float __attribute__((ext_vector_type(4)));
+    using lds_v4f = __attribute__((address_space(3))) v4f;
     __shared__ int s_b[kChunkMaxB + 1];
     __shared__ int s_nc;
     __shared__ int s_ctr;
@@ -1645,12 +1647,13 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     extern __shared__ float s_part_dyn[];
     float(*s_part)[3][64] = reinterpret_cast<float(*)[3][64]>(s_part_dyn);
     __shared__ int s_vis[64];
-    // LW: each wave's window of (com, G m, accept threshold) sits in LDS,
-    // one 64-float row per component, and a step reads its record with
-    // broadcast loads (one address, row offsets) instead of five v_readlane;
-    // the skip slot stays in a register (v_readlane): it feeds the scalar
-    // choice of the next slot
-    __shared__ float s_win[LW ? NW : 1][5][64];
+    // LW: each wave's window of records sits in LDS as (com, accept
+    // threshold), 16 bytes a slot, and a step reads its record with one
+    // broadcast load instead of four v_readlane.  G m and the skip slot stay
+    // in registers (v_readlane): G m is needed only on accept, so an accept
+    // costs no second LDS round trip, and the skip feeds the scalar choice of
+    // the next slot.
+    __shared__ v4f s_rec[LW ? NW : 1][64];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int g = xcd_run_block(blockIdx.x, gridDim.x, cs.run);
@@ -1696,7 +1699,40 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         pz = self.z;
     }
     __syncthreads();
-    const int nc = s_nc;
+    const int nc = __builtin_amdgcn_readfirstlane(s_nc);
+    // The window [k0, k0 + 64): lane l holds record k0 + l in wa / wb (and,
+    // LW, its com and threshold in LDS; rb is the LDS byte address of slot 0
+    // in the window's frame).  It carries over from chunk to chunk; a chunk's
+    // walk starts again at the root, below k0, so every window check is
+    // unsigned.
+    const unsigned rec0 =
+        LW ? static_cast<unsigned>((size_t)(const lds_v4f *)&s_rec[LW ? wave : 0][0]) : 0u;
+    unsigned rb = 0;
+    int k0 = -64;
+    float4 wa = make_float4(0.f, 0.f, 0.f, 0.f);
+    int2 wb = make_int2(0, 0);
+    const int lm = m - 1;
+    // Loads the window at slot KK (slots past the last record read the last
+    // one, never used).
+#define RG_WALK7_REFILL(KK)                                                   \
+    do {                                                                      \
+        k0 = (KK);                                                            \
+        const int sl_ = min(k0 + lane, lm);                                   \
+        wa = recA[sl_];                                                       \
+        wb = recB[sl_];                                                       \
+        if (LW) {                                                             \
+            v4f w_;                                                           \
+            w_.x = wa.x;                                                      \
+            w_.y = wa.y;                                                      \
+            w_.z = wa.z;                                                      \
+            w_.w = __int_as_float(wb.x);                                      \
+            s_rec[LW ? wave : 0][lane] = w_;                                  \
+            rb = rec0 - 16u * static_cast<unsigned>(k0);                      \
+            /* the window is read across lanes: reads stay after writes */    \
+            __builtin_amdgcn_wave_barrier();                                  \
+        }                                                                     \
+    } while (0)
+#define RG_WALK7_READ(KK) (*(const lds_v4f *)(size_t)(rb + 16u * static_cast<unsigned>(KK)))
     int wvis = 0;
     while (true) {
         int j = 0;
@@ -1704,7 +1740,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         j = __builtin_amdgcn_readfirstlane(__shfl(j, 0, 64));
         if (j >= nc) break;
         const int c = (j & 1) ? nc - 1 - (j >> 1) : (j >> 1);
-        const int lo = s_b[c], hi = s_b[c + 1];
+        const int lo = __builtin_amdgcn_readfirstlane(s_b[c]);
+        const int hi = __builtin_amdgcn_readfirstlane(s_b[c + 1]);
         float ax = 0.f, ay = 0.f, az = 0.f;
         int visits = 0;
         // A lane takes part at slot k when next <= k.  Accepting a node sets
@@ -1712,31 +1749,11 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         // k + 1), which saves a register write per step.
         int next = (active && lo < hi) ? 0 : m;
         int k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
-        int k0 = -64;
-        float4 wa = make_float4(0.f, 0.f, 0.f, 0.f);
-        int2 wb = make_int2(0, 0);
         // Ancestors of lo (slots before the chunk whose subtree reaches into
         // it): tested, not counted or added -- an earlier chunk owns them.
         // Subtrees ending at or before lo are stepped over untested.
         while (k < lo) {
-            if (k - k0 >= 64) {
-                k0 = k;
-                const int slot = k0 + lane;
-                if (slot < m) {
-                    wa = recA[slot];
-                    wb = recB[slot];
-                    if (LW) {
-                        float *row = &s_win[LW ? wave : 0][0][lane];
-                        row[0] = wa.x;
-                        row[64] = wa.y;
-                        row[128] = wa.z;
-                        row[192] = wa.w;
-                        row[256] = __int_as_float(wb.x);
-                    }
-                }
-                // the window is read across lanes: keep the reads after the writes
-                if (LW) __builtin_amdgcn_wave_barrier();
-            }
+            if (__builtin_expect(static_cast<unsigned>(k - k0) >= 64u, 0)) RG_WALK7_REFILL(k);
             const int l = k - k0;
             const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
             if (skip <= lo) {
@@ -1744,24 +1761,23 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             } else {
                 float cx, cy, cz, crit;
                 if (LW) {
-                    const float *row = &s_win[LW ? wave : 0][0][l];
-                    cx = row[0];
-                    cy = row[64];
-                    cz = row[128];
-                    crit = row[256];
+                    const v4f r = RG_WALK7_READ(k);
+                    cx = r.x;
+                    cy = r.y;
+                    cz = r.z;
+                    crit = r.w;
                 } else {
                     cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
                     cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
                     cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
                     crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
                 }
-                if (next <= k) {
-                    const float dx = cx - px;
-                    const float dy = cy - py;
-                    const float dz = cz - pz;
-                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-                    if (crit < r2) next = skip;  // opened: stays active (next <= k + 1)
-                }
+                // a select, not a branch: lanes not at k keep their next
+                const float dx = cx - px;
+                const float dy = cy - py;
+                const float dz = cz - pz;
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+                next = (next <= k && crit < r2) ? skip : next;  // opened: stays <= k + 1
             }
             // the smallest next slot: k + 1 if a lane opened k, else skip(k)
             // (every other lane waits at an ancestor's skip, >= skip(k))
@@ -1770,38 +1786,19 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         // The chunk's own slots: every node a lane reaches is tested, counted
         // and, when accepted, added.
         while (k < hi) {
-            if (k - k0 >= 64) {
-                k0 = k;
-                const int slot = k0 + lane;
-                if (slot < m) {
-                    wa = recA[slot];
-                    wb = recB[slot];
-                    if (LW) {
-                        float *row = &s_win[LW ? wave : 0][0][lane];
-                        row[0] = wa.x;
-                        row[64] = wa.y;
-                        row[128] = wa.z;
-                        row[192] = wa.w;
-                        row[256] = __int_as_float(wb.x);
-                    }
-                }
-                // the window is read across lanes: keep the reads after the writes
-                if (LW) __builtin_amdgcn_wave_barrier();
-            }
+            if (__builtin_expect(static_cast<unsigned>(k - k0) >= 64u, 0)) RG_WALK7_REFILL(k);
             const int l = k - k0;
-            float cx, cy, cz, cw, crit;
+            float cx, cy, cz, crit;
             if (LW) {
-                const float *row = &s_win[LW ? wave : 0][0][l];
-                cx = row[0];
-                cy = row[64];
-                cz = row[128];
-                cw = row[192];
-                crit = row[256];
+                const v4f r = RG_WALK7_READ(k);
+                cx = r.x;
+                cy = r.y;
+                cz = r.z;
+                crit = r.w;
             } else {
                 cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
                 cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
                 cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
-                cw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
                 crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
             }
             const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
@@ -1812,6 +1809,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 const float dz = cz - pz;
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
                 if (crit < r2) {
+                    const float cw =
+                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
                     const float inv = __builtin_amdgcn_rsqf(r2);
                     const float s = cw * (inv * inv) * inv;
                     ax = fmaf(s, dx, ax);
@@ -1856,6 +1855,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         const int mx = wave_max_i32(s_vis[lane]);
         if (lane == 0) atomicMax(&stats[1], static_cast<unsigned long long>(mx));
     }
+#undef RG_WALK7_READ
+#undef RG_WALK7_REFILL
 }
 
 
