@@ -1784,31 +1784,43 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             k = __ballot(next <= k + 1) ? k + 1 : skip;
         }
         // The chunk's own slots: every node a lane reaches is tested, counted
-        // and, when accepted, added.
-        while (k < hi) {
-            if (__builtin_expect(static_cast<unsigned>(k - k0) >= 64u, 0)) RG_WALK7_REFILL(k);
-            const int l = k - k0;
-            float cx, cy, cz, crit;
-            if (LW) {
-                const v4f r = RG_WALK7_READ(k);
-                cx = r.x;
-                cy = r.y;
-                cz = r.z;
-                crit = r.w;
-            } else {
-                cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
-                cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
-                cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
-                crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
-            }
-            const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
-            if (next <= k) {
-                ++visits;
+        // and, when accepted, added.  The window check is at the end of a
+        // step, on the next slot, whose window offset l it leaves for the
+        // next step.
+        if (k < hi) {
+            if (static_cast<unsigned>(k - k0) >= 64u) RG_WALK7_REFILL(k);
+            int l = k - k0;
+            do {
+                float cx, cy, cz, crit;
+                if (LW) {
+                    const v4f r = RG_WALK7_READ(k);
+                    cx = r.x;
+                    cy = r.y;
+                    cz = r.z;
+                    crit = r.w;
+                } else {
+                    cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
+                    cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
+                    cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
+                    crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+                }
+                const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
+                // the test and the next-slot update are selects; only the
+                // force is under a branch (lanes that accept)
+                const bool act = next <= k;
+                visits += act ? 1 : 0;
                 const float dx = cx - px;
                 const float dy = cy - py;
                 const float dz = cz - pz;
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-                if (crit < r2) {
+                const bool acc = act && crit < r2;
+                next = acc ? skip : next;  // opened: stays active (next <= k + 1)
+                // the next slot is chosen before the force (the empty asm
+                // keeps its scalar select there), so the select's latency
+                // overlaps the force's arithmetic
+                const int kn = __ballot(next <= k + 1) ? k + 1 : skip;
+                asm volatile("" ::"s"(kn));
+                if (acc) {
                     const float cw =
                         __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
                     const float inv = __builtin_amdgcn_rsqf(r2);
@@ -1816,10 +1828,16 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     ax = fmaf(s, dx, ax);
                     ay = fmaf(s, dy, ay);
                     az = fmaf(s, dz, az);
-                    next = skip;
-                }  // opened: stays active (next <= k + 1)
-            }
-            k = __ballot(next <= k + 1) ? k + 1 : skip;
+                }
+                l = kn - k0;
+                if (__builtin_expect(static_cast<unsigned>(l) >= 64u, 0)) {
+                    if (kn < hi) {
+                        RG_WALK7_REFILL(kn);
+                        l = 0;
+                    }
+                }
+                k = kn;
+            } while (k < hi);
         }
         s_part[c][0][lane] = ax;
         s_part[c][1][lane] = ay;
