@@ -1748,7 +1748,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         // next to its skip slot; opening it leaves next alone (it stays <=
         // k + 1), which saves a register write per step.
         int next = (active && lo < hi) ? 0 : m;
-        int k = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(next));
+        // the smallest next: 0 when any lane walks this chunk, else m
+        int k = __ballot(next == 0) ? 0 : m;
         // Ancestors of lo (slots before the chunk whose subtree reaches into
         // it): tested, not counted or added -- an earlier chunk owns them.
         // Subtrees ending at or before lo are stepped over untested.
