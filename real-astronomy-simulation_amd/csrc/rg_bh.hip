@@ -103,8 +103,8 @@ __device__ __forceinline__ void block_box_reduce(double v[7], double (*s_w)[7])
 // Statistics words of a compute (device, 8 x u64): [0] nodes visited by the
 // walk, [1] the most one particle visited, [2] non-finite coordinates,
 // [3] look-back waits that gave up (radix sort, treelet ranking: the order
-// and tree are then invalid), [4] walks or preorder chains that ran out of
-// bounds (a corrupt tree), [6] sample-sort buckets sorted in global memory.
+// and tree are then invalid), [4] FP64 walks that ran out of bounds (a
+// corrupt tree), [6] sample-sort buckets sorted in global memory.
 constexpr int kStatWords = 8;
 
 // A look-back wait of this compute gave up (stats[3]): every kernel after
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kBoxBS) void k_morton(const double *__restrict__ po
                                                    uint64_t *__restrict__ keys, int passes,
                                                    unsigned *__restrict__ ghist,
                                                    unsigned *__restrict__ lookback, int lb_words,
-                                                   int *__restrict__ tl_of)
+                                                   int2 *__restrict__ tl_of)
 {
     __shared__ unsigned s_h[kMaxPasses * kBuckets];
     __shared__ double s_w[kBoxBS / 64][7];
@@ -290,7 +290,9 @@ __global__ __launch_bounds__(kBoxBS) void k_morton(const double *__restrict__ po
             key |= part1by2(static_cast<uint64_t>(t)) << a;
         }
         keys[i] = key;
-        if (tl_of) tl_of[i] = -1;  // set again by the build for the first leaf of each treelet
+        // set again by the build: the treelet starting here, and the count of
+        // internal nodes whose leaf range ends here
+        if (tl_of) tl_of[i] = make_int2(-1, 0);
         for (int p = 0; p < passes; ++p)
             atomicAdd(&s_h[p * kBuckets + ((key >> (kDigitBits * p)) & kDigitMask)], 1u);
     }
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(kBoxBS) void k_morton_sample(
     const uint64_t *__restrict__ prev_keys, const int *__restrict__ prev_idx, int stride,
     int nbuckets, uint8_t *__restrict__ bkt,
     unsigned *__restrict__ ghist, unsigned *__restrict__ lookback, int lb_words,
-    int *__restrict__ tl_of)
+    int2 *__restrict__ tl_of)
 {
     static_assert(kBoxBS == kBuckets, "one thread per splitter slot and per bucket");
     __shared__ unsigned s_h[kBuckets];
@@ -394,7 +396,9 @@ __global__ __launch_bounds__(kBoxBS) void k_morton_sample(
     for (int i = blockIdx.x * kBoxBS + t; i < n; i += gridDim.x * kBoxBS) {
         const uint64_t key = morton_key_at(pos, i, lo, scale, qmax);
         keys[i] = key;
-        if (tl_of) tl_of[i] = -1;  // set again by the build for the first leaf of each treelet
+        // set again by the build: the treelet starting here, and the count of
+        // internal nodes whose leaf range ends here
+        if (tl_of) tl_of[i] = make_int2(-1, 0);
         // splitters <= (key, i): at most nbuckets - 1 <= 255 of the 256 slots
         int c = 0;
 #pragma unroll
@@ -430,7 +434,7 @@ struct SampleSort {
 
 int launch_bbox_morton(const double *pos, int n, int bits, double *part, double *box,
                        unsigned long long *stats, uint64_t *keys, int passes, unsigned *ghist,
-                       int *tickets, unsigned *lookback, int lb_words, int *counts, int *tl_of,
+                       int *tickets, unsigned *lookback, int lb_words, int *counts, int2 *tl_of,
                        hipStream_t st, const KickDrift *kd = nullptr,
                        const SampleSort *ss = nullptr)
 {
@@ -479,29 +483,28 @@ __device__ __forceinline__ int delta_dev(int64_t i, int64_t j, int64_t n, uint64
     return __clzll(static_cast<long long>(x)) - 1;
 }
 
-// Upward links for the preorder pass: up[v] = (parent id << 1) | (v is the
-// right child); up[root] = -1.
 constexpr int kTreelet = 256;  // max leaves of a treelet (one workgroup, LDS)
 
-// The Karras build, plus the aggregation plan.  Treelets are the maximal
-// subtrees with at most kTreelet leaves (a treelet root has at most kTreelet
-// leaves and its parent more); they partition the leaves.  The thread of
-// internal node i knows the leaf counts of i and of both children, so it
-// marks each child that is a treelet root: tl_of[first leaf of a treelet] =
-// its root (tl_of was cleared to -1 by k_morton).  The ranking pass then
-// lists the treelets in leaf order.  Also clears its look-back words.
+// The Karras build, plus the aggregation plan and the preorder counts.
+// Treelets are the maximal subtrees with at most kTreelet leaves (a treelet
+// root has at most kTreelet leaves and its parent more); they partition the
+// leaves.  The thread of internal node i knows the leaf counts of i and of
+// both children, so it marks each child that is a treelet root:
+// tl_of[first leaf of a treelet].x = its root.  It also counts itself at
+// the last leaf of its range, tl_of[hi].y (the ranking pass scans these
+// counts into preorder slots).  Both fields were cleared by the key pass.
+// Also clears the ranking pass's look-back words.
 __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restrict__ left,
                         int *__restrict__ right, int *__restrict__ parent,
-                        int *__restrict__ up, int2 *__restrict__ range,
-                        int *__restrict__ left_at_split, int *__restrict__ tl_of,
-                        unsigned *__restrict__ rank_lb, int rank_tiles,
-                        const unsigned long long *__restrict__ stats)
+                        int2 *__restrict__ range, int *__restrict__ left_at_split,
+                        int2 *__restrict__ tl_of, unsigned long long *__restrict__ rank_lb,
+                        int rank_tiles, const unsigned long long *__restrict__ stats)
 {
     if (tree_aborted(stats)) return;
     const int64_t nn = n;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < rank_tiles;
          k += gridDim.x * blockDim.x)
-        rank_lb[k] = 0u;
+        rank_lb[k] = 0ull;
     for (int ii = blockIdx.x * blockDim.x + threadIdx.x; ii < n - 1;
          ii += gridDim.x * blockDim.x) {
         const int64_t i = ii;
@@ -532,48 +535,33 @@ __global__ void k_build(const uint64_t *__restrict__ code, int n, int *__restric
         right[i] = rc;
         parent[lc] = self;
         parent[rc] = self;
-        up[lc] = self << 1;
-        up[rc] = (self << 1) | 1;
         range[i] = make_int2(static_cast<int>(lo), static_cast<int>(hi));
         left_at_split[gamma] = lc;
-        if (i == 0) {
-            parent[nn] = -1;
-            up[nn] = -1;
-        }
+        if (i == 0) parent[nn] = -1;
         if (hi - lo + 1 > kTreelet) {
-            if (gamma - lo + 1 <= kTreelet) tl_of[lo] = lc;
-            if (hi - gamma <= kTreelet) tl_of[gamma + 1] = rc;
+            if (gamma - lo + 1 <= kTreelet) tl_of[lo].x = lc;
+            if (hi - gamma <= kTreelet) tl_of[gamma + 1].x = rc;
         } else if (i == 0) {  // the root is itself a treelet
-            tl_of[0] = self;
+            tl_of[0].x = self;
         }
+        atomicAdd(&tl_of[hi].y, 1);
     }
 }
 
-// Right-first DFS preorder index of every node (the order the reference's
-// stack walk pops nodes in): pre(v) = 2 (n-1-hi(v)) + (right turns on the
-// root-to-v path), hi = last leaf of v's range.  The subtree of v then
-// occupies [pre(v), pre(v) + 2 (hi-lo+1) - 1).  stats[4] flags a chain
-// longer than any valid tree (corrupt links).
-constexpr int kMaxDepth = 128;  // > 63 + 31 + 1, the deepest tree the tie-break allows
-
-__device__ void preorder_part(const int *__restrict__ up, const int2 *__restrict__ range, int n,
-                              int *__restrict__ pre, unsigned long long *__restrict__ stats,
-                              int block, int nblocks)
-{
-    const int m = 2 * n - 1;
-    for (int v = block * blockDim.x + threadIdx.x; v < m; v += nblocks * blockDim.x) {
-        const int hi = v < n ? v : range[v - n].y;
-        int r = 0, u = v, steps = 0;
-        while (u != n && u >= 0 && u < m && steps < kMaxDepth) {
-            const int x = up[u];
-            r += x & 1;
-            u = x >> 1;
-            ++steps;
-        }
-        if (u != n) atomicAdd(&stats[4], 1ull);
-        pre[v] = 2 * (n - 1 - hi) + r;
-    }
-}
+// Right-first DFS preorder slots (the order the reference's stack walk pops
+// nodes in).  A node v with leaf range [lo, hi] is preceded by the nodes of
+// the subtrees right of it, 2 (n-1-hi) - (their count), and by its
+// ancestors; each ancestor where the path turns left roots one of those
+// subtrees, so pre(v) = 2 (n-1-hi) + R(v), R the right turns on the
+// root-to-v path.  The right children containing leaf lo are exactly the
+// nodes of the path (and of v's leftmost descent, all left children) where
+// it turns right, and each internal node a has one right child, with range
+// [split(a) + 1, hi(a)].  Every leaf position p >= 1 is split(a) + 1 for
+// exactly one a, so R(v) = lo - F(lo), F(x) the number of internal nodes
+// whose range ends before x: an exclusive scan of the per-leaf counts that
+// k_build leaves in tl_of[.].y.  For a leaf, pre(p) = 2 (n-1-p) + p - F(p);
+// an internal node's subtree ends with its leftmost leaf, so
+// pre(v) = pre(leaf lo) - 2 (hi - lo).
 
 // --------------------------------------------------------- aggregation
 
@@ -643,19 +631,26 @@ __device__ __forceinline__ void write_node(Agg *agg, const WalkOut &wo,
 constexpr int kRankItems = 16;
 constexpr int kRankTile = 256 * kRankItems;  // leaves per treelet-rank tile
 
-// Rank of every treelet in leaf order: an exclusive scan over the leaves of
-// "first leaf of a treelet", one tile of kRankTile leaves per workgroup with
-// decoupled look-back (the sort's scheme: one word per tile, status + count,
-// bounded waits).  trank[first leaf] = rank, tsorted[rank] = root, and
-// counts[3] = the number of treelets.
-__device__ void treelet_rank_tile(const int *__restrict__ tl_of, int n, int *__restrict__ trank,
-                                  int *__restrict__ tsorted, unsigned *__restrict__ lb,
-                                  int *__restrict__ counts, int ntiles,
-                                  unsigned long long *__restrict__ stats, int spin_limit)
+// Rank of every treelet in leaf order, and the preorder slot of every leaf:
+// two exclusive scans over the leaf positions done together -- of "first
+// leaf of a treelet" (tl_of[i].x >= 0) and of the internal-node range-end
+// counts (tl_of[i].y) -- one tile of kRankTile leaves per workgroup with
+// decoupled look-back (the sort's scheme: one 64-bit word per tile, status +
+// the two counts side by side, bounded waits).  trank[first leaf] = rank,
+// tsorted[rank] = root, counts[3] = the number of treelets, and (pre set)
+// pre[leaf] = its preorder slot.
+__device__ void treelet_rank_tile(const int2 *__restrict__ tl_of, int n, int *__restrict__ trank,
+                                  int *__restrict__ tsorted, int *__restrict__ pre,
+                                  unsigned long long *__restrict__ lb, int *__restrict__ counts,
+                                  int ntiles, unsigned long long *__restrict__ stats,
+                                  int spin_limit)
 {
+    using W = unsigned long long;
+    constexpr int kHi = 31;  // the range-end count's field
+    constexpr W kLo = (1ull << kHi) - 1ull;
     __shared__ int s_tile;
-    __shared__ unsigned s_wsum[4];
-    __shared__ unsigned s_excl;
+    __shared__ W s_wsum[4];
+    __shared__ W s_excl;
     const int t = threadIdx.x;
     const int w = t >> 6;
     const int lane = t & 63;
@@ -664,67 +659,70 @@ __device__ void treelet_rank_tile(const int *__restrict__ tl_of, int n, int *__r
     const int tile = s_tile;
     const int base = tile * kRankTile + t * kRankItems;
     unsigned bits = 0;
+    int root[kRankItems], ends[kRankItems];
+    unsigned esum = 0;
 #pragma unroll
     for (int r = 0; r < kRankItems; ++r) {
         const int i = base + r;
-        if (i < n && tl_of[i] >= 0) bits |= 1u << r;
+        const int2 e = i < n ? tl_of[i] : make_int2(-1, 0);
+        root[r] = e.x;
+        ends[r] = e.y;
+        if (e.x >= 0) bits |= 1u << r;
+        esum += static_cast<unsigned>(e.y);
     }
-    const unsigned cnt = __popc(bits);
-    unsigned incl = cnt;
+    const W mine = (static_cast<W>(esum) << kHi) | __popc(bits);
+    W incl = mine;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const unsigned v = __shfl_up(incl, off, 64);
+        const W v = __shfl_up(incl, off, 64);
         if (lane >= off) incl += v;
     }
     if (lane == 63) s_wsum[w] = incl;
     __syncthreads();
-    unsigned woff = 0, total = 0;
+    W woff = 0, total = 0;
 #pragma unroll
     for (int ww = 0; ww < 4; ++ww) {
         if (ww < w) woff += s_wsum[ww];
         total += s_wsum[ww];
     }
     if (t == 0) {
-        unsigned *mine = &lb[tile];
-        __hip_atomic_store(mine, (tile == 0 ? kFlagInc : kFlagAgg) | total, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        unsigned excl = 0;
+        W *slot = &lb[tile];
+        __hip_atomic_store(slot, (tile == 0 ? LbWord<W>::inc : LbWord<W>::agg) | total,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        W excl = 0;
         if (tile > 0) {
-            excl = look_back(lb, tile, 1, stats, spin_limit);
-            __hip_atomic_store(mine, kFlagInc | (excl + total), __ATOMIC_RELAXED,
+            excl = look_back<kLookWin, W>(lb, tile, 1, stats, spin_limit);
+            __hip_atomic_store(slot, LbWord<W>::inc | (excl + total), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
         s_excl = excl;
-        if (tile == ntiles - 1) counts[3] = static_cast<int>(excl + total);
+        if (tile == ntiles - 1) counts[3] = static_cast<int>((excl + total) & kLo);
     }
     __syncthreads();
-    unsigned r0 = s_excl + woff + incl - cnt;
+    const W start = s_excl + woff + incl - mine;
+    unsigned r0 = static_cast<unsigned>(start & kLo);
+    int f = static_cast<int>(start >> kHi);  // internal nodes ending before leaf base
 #pragma unroll
     for (int r = 0; r < kRankItems; ++r) {
+        const int i = base + r;
         if ((bits >> r) & 1u) {
-            const int i = base + r;
             trank[i] = static_cast<int>(r0);
-            tsorted[r0] = tl_of[i];
+            tsorted[r0] = root[r];
             ++r0;
         }
+        if (pre && i < n) pre[i] = 2 * (n - 1 - i) + i - f;
+        f += ends[r];
     }
 }
 
-// The first ntiles blocks rank the treelets; the others (when pre is set)
-// compute the preorder slots -- two independent passes in one launch.  A
-// ranking block waits only on ranking blocks that took earlier tickets.
+// One workgroup per ranking tile.
 __global__ __launch_bounds__(256) void k_rank_preorder(
-    const int *__restrict__ tl_of, int n, int *__restrict__ trank, int *__restrict__ tsorted,
-    unsigned *__restrict__ lb, int *__restrict__ counts, int ntiles,
-    unsigned long long *__restrict__ stats, const int *__restrict__ up,
-    const int2 *__restrict__ range, int *__restrict__ pre, int spin_limit)
+    const int2 *__restrict__ tl_of, int n, int *__restrict__ trank, int *__restrict__ tsorted,
+    int *__restrict__ pre, unsigned long long *__restrict__ lb, int *__restrict__ counts,
+    int ntiles, unsigned long long *__restrict__ stats, int spin_limit)
 {
     if (tree_aborted(stats)) return;
-    if (static_cast<int>(blockIdx.x) < ntiles) {
-        treelet_rank_tile(tl_of, n, trank, tsorted, lb, counts, ntiles, stats, spin_limit);
-        return;
-    }
-    preorder_part(up, range, n, pre, stats, blockIdx.x - ntiles, gridDim.x - ntiles);
+    treelet_rank_tile(tl_of, n, trank, tsorted, pre, lb, counts, ntiles, stats, spin_limit);
 }
 
 // Pass 1: one workgroup per treelet (grid-stride over the list).  Every
@@ -761,6 +759,7 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
     __shared__ double s_mn[3][C];
     __shared__ double s_mx[3][C];
     __shared__ int s_done[C];       // pass that finished internal node lo + t (0: not yet)
+    __shared__ int s_pre[MODE != 0 ? C : 1];  // leaves' preorder slots
     const int tid = threadIdx.x;
     const int cnt = counts[3];  // treelets, in leaf order in tsorted
     for (int e = blockIdx.x; e < cnt; e += gridDim.x) {
@@ -777,7 +776,6 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
             rg = range[ii];
             L = left[ii];
             R = right[ii];
-            if (MODE != 0) npre = wo.pre[ii + n];
             own = rg.x >= lo && rg.y <= hi;  // inside this treelet
         }
         s_done[tid] = 0;
@@ -794,10 +792,15 @@ __global__ __launch_bounds__(kTreelet) void k_agg_treelet(
                 s_lx[a][tid] = x[a];
             }
             s_lm[tid] = m;
+            const int lpre = MODE != 0 ? wo.pre[li] : 0;
+            if (MODE != 0) s_pre[tid] = lpre;
             write_node<MODE>(agg, wo, li, n, m, mom, x, x, x, li, li, -1,
-                             li == 0 ? -1 : left_at_split[li - 1], MODE != 0 ? wo.pre[li] : 0);
+                             li == 0 ? -1 : left_at_split[li - 1], lpre);
         }
         __syncthreads();
+        // an internal node's slot from its leftmost leaf's (its subtree's
+        // last slot): pre(v) = pre(lo) - 2 (hi - lo)
+        if (MODE != 0 && own) npre = s_pre[rg.x - lo] - 2 * (rg.y - rg.x);
         if (v >= n) {  // uniform: the treelet has internal nodes
             const int troot = v - n - lo;
             double mm = 0.0, mom[3] = {0.0, 0.0, 0.0}, mn[3] = {0.0, 0.0, 0.0},
@@ -972,7 +975,7 @@ __global__ __launch_bounds__(256) void k_agg_upper(
             }
             write_node<MODE>(agg, wo, v, n, mm, mom, mn, mx, cm, rg.x, rg.y, right[ii],
                              rg.x == 0 ? -1 : left_at_split[rg.x - 1],
-                             MODE == 0 ? 0 : wo.pre[v]);
+                             MODE == 0 ? 0 : wo.pre[rg.x] - 2 * (rg.y - rg.x));
         }
         __syncthreads();  // s_r is reused by the next node
     }
@@ -2158,9 +2161,10 @@ struct rg_bh {
     int *left = nullptr, *right = nullptr, *parent = nullptr;
     int2 *range = nullptr;
     int *left_at_split = nullptr;
-    int *up = nullptr, *pre = nullptr;
-    int *counts = nullptr, *tl_of = nullptr, *trank = nullptr, *tsorted = nullptr;
-    unsigned *rank_lb = nullptr;  // treelet-rank look-back words, one per tile
+    int *pre = nullptr;  // FP32: the leaves' preorder slots
+    int *counts = nullptr, *trank = nullptr, *tsorted = nullptr;
+    int2 *tl_of = nullptr;  // per leaf: treelet starting there, internal nodes ending there
+    unsigned long long *rank_lb = nullptr;  // treelet-rank look-back words, one per tile
     int rank_tiles = 0;
     Agg *agg = nullptr;
     WalkRec64 *rec64 = nullptr;  // FP64 walk records (node-id order)
@@ -2305,18 +2309,17 @@ extern "C" int rg_bh_create(int64_t n, double theta, double eps, double G, int m
     ALLOC(h->range, 8 * (N - 1));
     ALLOC(h->left_at_split, 4 * (N - 1));
     ALLOC(h->parent, 4 * M);
-    ALLOC(h->up, 4 * M);
     ALLOC(h->counts, 16);
-    ALLOC(h->tl_of, 4 * N);
+    ALLOC(h->tl_of, 8 * N);
     ALLOC(h->trank, 4 * N);
     ALLOC(h->tsorted, 4 * N);
     h->rank_tiles = static_cast<int>((N + kRankTile - 1) / kRankTile);
-    ALLOC(h->rank_lb, 4 * static_cast<size_t>(h->rank_tiles));
+    ALLOC(h->rank_lb, 8 * static_cast<size_t>(h->rank_tiles));
     ALLOC(h->agg, sizeof(Agg) * M);
     if (precision == RG_FP64) {
         ALLOC(h->rec64, sizeof(WalkRec64) * M);
     } else {
-        ALLOC(h->pre, 4 * M);
+        ALLOC(h->pre, 4 * N);
         ALLOC(h->recA, 16 * M);
         ALLOC(h->recB, 8 * M);
     }
@@ -2453,18 +2456,15 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
     const int *order = h->idx[cur];
 
     const int g1 = min((n + 255) / 256, 16384);
-    k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->up, h->range,
+    k_build<<<g1, 256, 0, st>>>(code, n, h->left, h->right, h->parent, h->range,
                                 h->left_at_split, h->tl_of, h->rank_lb, h->rank_tiles, h->stats);
     RG_CHECK_LAUNCH();
     const bool fp64 = h->precision == RG_FP64;
-    {
-        // FP32 walks also need the preorder slots (blocks after the ranking ones)
-        const int gp = fp64 ? 0 : min((2 * n - 1 + 255) / 256, 16384);
-        k_rank_preorder<<<h->rank_tiles + gp, 256, 0, st>>>(
-            h->tl_of, n, h->trank, h->tsorted, h->rank_lb, h->counts, h->rank_tiles, h->stats,
-            h->up, h->range, h->pre, rg::g_spin_limit);
-        RG_CHECK_LAUNCH();
-    }
+    // FP32 walks also need the leaves' preorder slots
+    k_rank_preorder<<<h->rank_tiles, 256, 0, st>>>(h->tl_of, n, h->trank, h->tsorted,
+                                                   fp64 ? nullptr : h->pre, h->rank_lb, h->counts,
+                                                   h->rank_tiles, h->stats, rg::g_spin_limit);
+    RG_CHECK_LAUNCH();
 
     const double theta2 = h->theta * h->theta;
     WalkOut wo;
@@ -2711,7 +2711,7 @@ extern "C" int rg_bh_stats(rg_bh *h, rg_bh_stats_t *out)
         return RG_ERR_CAPACITY;
     }
     RG_REQUIRE(s[4] == 0, RG_ERR_CAPACITY,
-               "walk or preorder pass ran out of bounds for %llu nodes (corrupt tree)", s[4]);
+               "FP64 walk ran out of bounds for %llu particles (corrupt tree)", s[4]);
     return RG_OK;
 }
 

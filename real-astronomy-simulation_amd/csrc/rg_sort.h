@@ -79,6 +79,22 @@ constexpr unsigned kFlagAgg = 1u << 30;
 constexpr unsigned kFlagInc = 2u << 30;
 constexpr unsigned kCountMask = (1u << 30) - 1u;
 
+// The status and count fields of a look-back word: 32-bit words as above;
+// 64-bit words keep the status in the top two bits and a 62-bit count (two
+// 31-bit counts side by side add as one while neither total reaches 2^31).
+template <typename W>
+struct LbWord {
+    static constexpr W agg = kFlagAgg;
+    static constexpr W inc = kFlagInc;
+    static constexpr W mask = kCountMask;
+};
+template <>
+struct LbWord<unsigned long long> {
+    static constexpr unsigned long long agg = 1ull << 62;
+    static constexpr unsigned long long inc = 2ull << 62;
+    static constexpr unsigned long long mask = (1ull << 62) - 1ull;
+};
+
 // Exclusive prefix over earlier tiles of one look-back word column.  The
 // words of the kLookWin tiles before the current position are loaded
 // together (independent loads, one memory round trip) and consumed in tile
@@ -91,36 +107,35 @@ constexpr unsigned kCountMask = (1u << 30) - 1u;
 // device-scope loads than they save round trips).
 constexpr int kLookWin = 8;
 
-template <int WIN = kLookWin>
-__device__ __forceinline__ unsigned look_back(const unsigned *__restrict__ lb, int tile,
-                                              int stride, unsigned long long *__restrict__ stats,
-                                              int spin_limit)
+template <int WIN = kLookWin, typename W = unsigned>
+__device__ __forceinline__ W look_back(const W *__restrict__ lb, int tile, int stride,
+                                       unsigned long long *__restrict__ stats, int spin_limit)
 {
     constexpr int kLookWin = WIN;
-    unsigned excl = 0;
+    W excl = 0;
     int j = tile - 1;
     int spins = 0;
     if (spin_limit <= 0) {  // host-forced give-up (tests of the failure path)
         atomicAdd(&stats[3], 1ull);
-        return 0u;
+        return 0;
     }
     while (j >= 0) {
-        unsigned v[kLookWin];
+        W v[kLookWin];
 #pragma unroll
         for (int q = 0; q < kLookWin; ++q)
             v[q] = j - q >= 0 ? __hip_atomic_load(&lb[(j - q) * stride], __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT)
-                              : kFlagInc;  // before tile 0: an inclusive zero
+                              : LbWord<W>::inc;  // before tile 0: an inclusive zero
         int used = 0;
         bool done = false;
 #pragma unroll
         for (int q = 0; q < kLookWin; ++q) {
             if (done || used < q) continue;  // stopped earlier in this window
-            const unsigned f = v[q] & ~kCountMask;
-            if (f == 0u) continue;           // not published: retry from here
-            excl += v[q] & kCountMask;
+            const W f = v[q] & ~LbWord<W>::mask;
+            if (f == 0) continue;            // not published: retry from here
+            excl += v[q] & LbWord<W>::mask;
             ++used;
-            if (f == kFlagInc) done = true;
+            if (f == LbWord<W>::inc) done = true;
         }
         if (done) break;
         j -= used;
