@@ -1736,7 +1736,6 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         }                                                                     \
     } while (0)
 #define RG_WALK7_READ(KK) (*(const lds_v4f *)(size_t)(rb + 16u * static_cast<unsigned>(KK)))
-    int wvis = 0;
     while (true) {
         int j = 0;
         if (lane == 0) j = atomicAdd(&s_ctr, 1);
@@ -1746,7 +1745,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         const int lo = __builtin_amdgcn_readfirstlane(s_b[c]);
         const int hi = __builtin_amdgcn_readfirstlane(s_b[c + 1]);
         float ax = 0.f, ay = 0.f, az = 0.f;
-        int visits = 0;
+        int accepts = 0;
         // A lane takes part at slot k when next <= k.  Accepting a node sets
         // next to its skip slot; opening it leaves next alone (it stays <=
         // k + 1), which saves a register write per step.
@@ -1812,7 +1811,6 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 // the test and the next-slot update are selects; only the
                 // force is under a branch (lanes that accept)
                 const bool act = next <= k;
-                visits += act ? 1 : 0;
                 const float dx = cx - px;
                 const float dy = cy - py;
                 const float dz = cz - pz;
@@ -1832,6 +1830,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     ax = fmaf(s, dx, ax);
                     ay = fmaf(s, dy, ay);
                     az = fmaf(s, dz, az);
+                    ++accepts;
                 }
                 l = kn - k0;
                 if (__builtin_expect(static_cast<unsigned>(l) >= 64u, 0)) {
@@ -1846,12 +1845,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         s_part[c][0][lane] = ax;
         s_part[c][1][lane] = ay;
         s_part[c][2][lane] = az;
-        if (visits) atomicAdd(&s_vis[lane], visits);
-        wvis += visits;
-    }
-    {
-        const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(wvis));
-        if (lane == 0) atomicAdd(&stats[0], tot);
+        if (accepts) atomicAdd(&s_vis[lane], accepts);
     }
     // No block barrier: a wave that runs out of chunks leaves at once (its
     // slot can take another block's wave), and the last wave to finish adds
@@ -1874,8 +1868,17 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             store_acc(out, p, static_cast<double>(ax), static_cast<double>(ay),
                       static_cast<double>(az));
         }
-        const int mx = wave_max_i32(s_vis[lane]);
-        if (lane == 0) atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+        // A lane's walk examines a full binary tree: every node it examines
+        // is accepted (leaves always are) or opened, and an opened node's two
+        // children are examined, so 2 a - 1 nodes for a accepted ones (over
+        // all chunks: the ancestor phases examine nothing of their own).
+        const int vis = active ? 2 * s_vis[lane] - 1 : 0;
+        const unsigned long long tot = wave_sum_u64(static_cast<unsigned long long>(vis));
+        const int mx = wave_max_i32(vis);
+        if (lane == 0) {
+            atomicAdd(&stats[0], tot);
+            atomicMax(&stats[1], static_cast<unsigned long long>(mx));
+        }
     }
 #undef RG_WALK7_READ
 #undef RG_WALK7_REFILL
