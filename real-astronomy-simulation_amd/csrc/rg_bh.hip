@@ -1723,6 +1723,9 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         const int sl_ = min(k0 + lane, lm);                                   \
         wa = recA[sl_];                                                       \
         wb = recB[sl_];                                                       \
+        /* a skip slot is past its own slot (in a valid tree it always is;  \
+           this keeps every walk finite whatever the records hold) */        \
+        wb.y = max(wb.y, k0 + lane + 1);                                      \
         if (LW) {                                                             \
             v4f w_;                                                           \
             w_.x = wa.x;                                                      \
@@ -1758,7 +1761,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         while (k < lo) {
             if (__builtin_expect(static_cast<unsigned>(k - k0) >= 64u, 0)) RG_WALK7_REFILL(k);
             const int l = k - k0;
-            const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
+            const int skip = __builtin_amdgcn_readlane(wb.y, l);
             if (skip <= lo) {
                 if (next <= k) next = skip;
             } else {
@@ -1807,7 +1810,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
                     crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
                 }
-                const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
+                const int skip = __builtin_amdgcn_readlane(wb.y, l);
                 // the test and the next-slot update are selects; only the
                 // force is under a branch (lanes that accept)
                 const bool act = next <= k;
