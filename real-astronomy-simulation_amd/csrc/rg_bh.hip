@@ -1811,21 +1811,27 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
                 }
                 const int skip = __builtin_amdgcn_readlane(wb.y, l);
-                // the test and the next-slot update are selects; only the
-                // force is under a branch (lanes that accept)
+                // The test is a select; only the force and the accepting
+                // lanes' next slot are under a branch.
                 const bool act = next <= k;
+                const unsigned long long mact = __ballot(act);
+                const unsigned long long mstay = __ballot(next <= k + 1);
                 const float dx = cx - px;
                 const float dy = cy - py;
                 const float dz = cz - pz;
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-                const bool acc = act && crit < r2;
-                next = acc ? skip : next;  // opened: stays active (next <= k + 1)
-                // the next slot is chosen before the force (the empty asm
-                // keeps its scalar select there), so the select's latency
-                // overlaps the force's arithmetic
-                const int kn = __ballot(next <= k + 1) ? k + 1 : skip;
+                const bool lt = crit < r2;
+                const unsigned long long macc = mact & __ballot(lt);
+                // The next slot, chosen before the force (the empty asm keeps
+                // its scalar select there, so its latency overlaps the force):
+                // k + 1 if a lane opened k, else skip(k).  A lane's next after
+                // this step is <= k + 1 when it did not accept and its next
+                // was <= k + 1, or when it accepted a node whose skip is k + 1
+                // (then both choices are k + 1), so the old next decides.
+                const int kn = (mstay & ~macc) ? k + 1 : skip;
                 asm volatile("" ::"s"(kn));
-                if (acc) {
+                if (act && lt) {
+                    next = skip;  // opened: stays active (next <= k + 1)
                     const float cw =
                         __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
                     const float inv = __builtin_amdgcn_rsqf(r2);
