@@ -1828,8 +1828,15 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 // this step is <= k + 1 when it did not accept and its next
                 // was <= k + 1, or when it accepted a node whose skip is k + 1
                 // (then both choices are k + 1), so the old next decides.
-                const int kn = (mstay & ~macc) ? k + 1 : skip;
-                asm volatile("" ::"s"(kn));
+                // (s_andn2 sets SCC when the result is nonzero: no compare)
+                int kn;
+                {
+                    unsigned long long t_;
+                    asm volatile("s_andn2_b64 %1, %2, %3\n\ts_cselect_b32 %0, %4, %5"
+                                 : "=s"(kn), "=&s"(t_)
+                                 : "s"(mstay), "s"(macc), "s"(k + 1), "s"(skip)
+                                 : "scc");
+                }
                 if (act && lt) {
                     next = skip;  // opened: stays active (next <= k + 1)
                     const float cw =
