@@ -1822,8 +1822,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
                 const bool lt = crit < r2;
                 const unsigned long long macc = mact & __ballot(lt);
-                // The next slot, chosen before the force (the empty asm keeps
-                // its scalar select there, so its latency overlaps the force):
+                // The next slot, chosen before the force (the asm keeps the
+                // scalar select there, so its latency overlaps the force):
                 // k + 1 if a lane opened k, else skip(k).  A lane's next after
                 // this step is <= k + 1 when it did not accept and its next
                 // was <= k + 1, or when it accepted a node whose skip is k + 1
