@@ -577,7 +577,9 @@ struct Agg {
 // right child and the escape (next node in right-first DFS order after the
 // subtree).  FP32: stored in right-first DFS preorder, so opening a node
 // moves to the next slot and accepting it skips its subtree: recA = (com,
-// G*mass), recB = (accept threshold, skip slot); leaves have threshold -inf.
+// accept threshold), recB = (G*mass bits, skip slot); leaves have threshold
+// -inf.  The 16-byte half is what every examined node needs; G*mass only an
+// accepted one.
 struct WalkOut {
     WalkRec64 *rec64;
     float4 *recA;
@@ -623,8 +625,9 @@ __device__ __forceinline__ void write_node(Agg *agg, const WalkOut &wo,
                                 : (wo.theta_zero ? INFINITY
                                                  : static_cast<float>(size * size * wo.inv_theta2));
         wo.recA[p] = make_float4(static_cast<float>(c[0]), static_cast<float>(c[1]),
-                                 static_cast<float>(c[2]), static_cast<float>(wo.G * m));
-        wo.recB[p] = make_int2(__float_as_int(crit), p + 2 * (hi - lo + 1) - 1);
+                                 static_cast<float>(c[2]), crit);
+        wo.recB[p] = make_int2(__float_as_int(static_cast<float>(wo.G * m)),
+                               p + 2 * (hi - lo + 1) - 1);
     }
 }
 
@@ -1136,9 +1139,9 @@ __global__ void k_walk32(const float4 *__restrict__ recA, const int2 *__restrict
             const float dy = c.y - py;
             const float dz = c.z - pz;
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-            if (__int_as_float(sb.x) < r2) {
+            if (c.w < r2) {
                 const float inv = __builtin_amdgcn_rsqf(r2);
-                const float s = c.w * (inv * inv) * inv;
+                const float s = __int_as_float(sb.x) * (inv * inv) * inv;
                 ax = fmaf(s, dx, ax);
                 ay = fmaf(s, dy, ay);
                 az = fmaf(s, dz, az);
@@ -1202,10 +1205,10 @@ __global__ void k_walk32_pair(const float4 *__restrict__ recA, const int2 *__res
             const float dy = c.y - py;
             const float dz = c.z - pz;
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-            if (__int_as_float(sb.x) < r2) {
+            if (c.w < r2) {
                 if (own) {
                     const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float s = c.w * (inv * inv) * inv;
+                    const float s = __int_as_float(sb.x) * (inv * inv) * inv;
                     ax = fmaf(s, dx, ax);
                     ay = fmaf(s, dy, ay);
                     az = fmaf(s, dz, az);
@@ -1283,8 +1286,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave(const float4 *__restrict__ 
         const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
         const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
         const float cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
-        const float cw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
-        const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+        const float cw = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+        const float crit = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
         const int skip = __builtin_amdgcn_readlane(wb.y, l);
         if (next == k) {
             ++visits;
@@ -1372,8 +1375,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave2(const float4 *__restrict__
             c[q][0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l + q));
             c[q][1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l + q));
             c[q][2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l + q));
-            c[q][3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l + q));
-            crit[q] = __int_as_float(__builtin_amdgcn_readlane(wb.x, l + q));
+            c[q][3] = __int_as_float(__builtin_amdgcn_readlane(wb.x, l + q));
+            crit[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l + q));
             skip[q] = __builtin_amdgcn_readlane(wb.y, l + q);
         }
         float dx[2], dy[2], dz[2], r2[2], s[2];
@@ -1500,8 +1503,8 @@ __global__ __launch_bounds__(256) void k_walk32_wave_split(
         const float cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
         const float cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
         const float cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
-        const float cw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
-        const float crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+        const float cw = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+        const float crit = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
         const int skip = max(__builtin_amdgcn_readlane(wb.y, l), k + 1);
         if (next == k) {
             if (skip <= lo) {
@@ -1720,9 +1723,11 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
 #define RG_WALK7_REFILL(KK)                                                   \
     do {                                                                      \
         k0 = (KK);                                                            \
-        const int sl_ = min(k0 + lane, lm);                                   \
-        wa = recA[sl_];                                                       \
-        wb = recB[sl_];                                                       \
+        /* the window's base is uniform (scalar address arithmetic); the   \
+           lanes add small 32-bit offsets */                                  \
+        const unsigned sl_ = static_cast<unsigned>(min(lane, lm - k0));       \
+        wa = (recA + k0)[sl_];                                                \
+        wb = (recB + k0)[sl_];                                                \
         /* a skip slot is past its own slot (in a valid tree it always is;  \
            this keeps every walk finite whatever the records hold) */        \
         wb.y = max(wb.y, k0 + lane + 1);                                      \
@@ -1731,7 +1736,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             w_.x = wa.x;                                                      \
             w_.y = wa.y;                                                      \
             w_.z = wa.z;                                                      \
-            w_.w = __int_as_float(wb.x);                                      \
+            w_.w = wa.w;  /* the record's 16-byte half as loaded */          \
             s_rec[LW ? wave : 0][lane] = w_;                                  \
             rb = rec0 - 16u * static_cast<unsigned>(k0);                      \
             /* the window is read across lanes: reads stay after writes */    \
@@ -1776,7 +1781,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
                     cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
                     cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
-                    crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+                    crit = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
                 }
                 // a select, not a branch: lanes not at k keep their next
                 const float dx = cx - px;
@@ -1808,7 +1813,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     cx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.x), l));
                     cy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.y), l));
                     cz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.z), l));
-                    crit = __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+                    crit = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
                 }
                 const int skip = __builtin_amdgcn_readlane(wb.y, l);
                 // The test is a select; only the force and the accepting
@@ -1840,7 +1845,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 if (act && lt) {
                     next = skip;  // opened: stays active (next <= k + 1)
                     const float cw =
-                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wa.w), l));
+                        __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
                     const float inv = __builtin_amdgcn_rsqf(r2);
                     const float s = cw * (inv * inv) * inv;
                     ax = fmaf(s, dx, ax);
@@ -2024,8 +2029,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked2(
                 row[0] = a.x;
                 row[64] = a.y;
                 row[128] = a.z;
-                row[192] = a.w;
-                row[256] = __int_as_float(r.x);
+                row[192] = __int_as_float(r.x);
+                row[256] = a.w;
                 w.skipw = r.y;
             }
             // the window is read across lanes: keep the reads after the writes
