@@ -122,7 +122,7 @@ const char *rg_last_error(void);
  *                 factor 2^shift of the next ones (0 = by n: 1 below 350k
  *                 particles, else 2); runs of groups per XCD (32); waves per
  *                 group, 4 or 8, or 2 for walk 8 (0 = by n: 8 below 90k
- *                 particles)
+ *                 particles, and below 300k with 10-bit keys)
  * Process-wide; takes effect at the next launch. */
 int rg_set_param(const char *name, int64_t value);
 

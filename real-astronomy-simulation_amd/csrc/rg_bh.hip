@@ -2542,10 +2542,13 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
         if (mode >= 6) {  // 6, 7 and 8 (the chunked walks)
             // Measured on disk + bulge galaxies, 60k .. 2M particles: octave
             // chunks (growth 2) below ~350k, then growth 4 (fewer ancestor
-            // tests once the walk is throughput-bound); 8 waves per group only
-            // pay below ~90k, where the groups are too few to fill the GPU.
+            // tests once the walk is throughput-bound); 8 waves per group pay
+            // below ~90k, where the groups are too few to fill the GPU, and
+            // below ~300k with 10-bit keys, whose equal-key runs in a dense
+            // centre make a few groups far heavier than the rest.
             const int shift = g_chunk_shift ? g_chunk_shift : (n < 350000 ? 1 : 2);
-            int waves = g_chunk_waves ? g_chunk_waves : (n < 90000 ? 8 : 4);
+            int waves = g_chunk_waves ? g_chunk_waves
+                                      : ((n < 90000 || (h->bits <= 10 && n < 300000)) ? 8 : 4);
             if (mode != 8 && waves == 2) waves = 4;  // two waves only for the two-walk kernel
             const ChunkShape cs{g_chunk_dmin, shift, g_chunk_run};
             const int ng = (n + 63) / 64;
