@@ -1723,11 +1723,9 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
 #define RG_WALK7_REFILL(KK)                                                   \
     do {                                                                      \
         k0 = (KK);                                                            \
-        /* the window's base is uniform (scalar address arithmetic); the   \
-           lanes add small 32-bit offsets */                                  \
-        const unsigned sl_ = static_cast<unsigned>(min(lane, lm - k0));       \
-        wa = (recA + k0)[sl_];                                                \
-        wb = (recB + k0)[sl_];                                                \
+        const int sl_ = min(k0 + lane, lm);                                   \
+        wa = recA[sl_];                                                       \
+        wb = recB[sl_];                                                       \
         /* a skip slot is past its own slot (in a valid tree it always is;  \
            this keeps every walk finite whatever the records hold) */        \
         wb.y = max(wb.y, k0 + lane + 1);                                      \
