@@ -1818,26 +1818,24 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 // lanes' next slot are under a branch.
                 const bool act = next <= k;
                 const unsigned long long mact = __ballot(act);
-                const unsigned long long mstay = __ballot(next <= k + 1);
                 const float dx = cx - px;
                 const float dy = cy - py;
                 const float dz = cz - pz;
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
                 const bool lt = crit < r2;
-                const unsigned long long macc = mact & __ballot(lt);
                 // The next slot, chosen before the force (the asm keeps the
                 // scalar select there, so its latency overlaps the force):
-                // k + 1 if a lane opened k, else skip(k).  A lane's next after
-                // this step is <= k + 1 when it did not accept and its next
-                // was <= k + 1, or when it accepted a node whose skip is k + 1
-                // (then both choices are k + 1), so the old next decides.
-                // (s_andn2 sets SCC when the result is nonzero: no compare)
+                // k + 1 if an active lane opened k, else skip(k).  No other
+                // lane can need k + 1: a lane waiting at k + 1 waits at the
+                // skip of an ancestor of k whose subtree ends at k, so k is a
+                // leaf, and then skip(k) = k + 1 as well.  (s_andn2 sets SCC
+                // when its result is nonzero: no compare.)
                 int kn;
                 {
                     unsigned long long t_;
                     asm volatile("s_andn2_b64 %1, %2, %3\n\ts_cselect_b32 %0, %4, %5"
                                  : "=s"(kn), "=&s"(t_)
-                                 : "s"(mstay), "s"(macc), "s"(k + 1), "s"(skip)
+                                 : "s"(mact), "s"(__ballot(lt)), "s"(k + 1), "s"(skip)
                                  : "scc");
                 }
                 if (act && lt) {
