@@ -1726,9 +1726,10 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         const int sl_ = min(k0 + lane, lm);                                   \
         wa = recA[sl_];                                                       \
         wb = recB[sl_];                                                       \
-        /* a skip slot is past its own slot (in a valid tree it always is;  \
-           this keeps every walk finite whatever the records hold) */        \
-        wb.y = max(wb.y, k0 + lane + 1);                                      \
+        /* the skip slot, relative to k0 and past its own slot (in a valid \
+           tree it always is; this keeps every walk finite whatever the      \
+           records hold) */                                                  \
+        wb.y = max(wb.y - k0, lane + 1);                                      \
         if (LW) {                                                             \
             v4f w_;                                                           \
             w_.x = wa.x;                                                      \
@@ -1764,7 +1765,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
         while (k < lo) {
             if (__builtin_expect(static_cast<unsigned>(k - k0) >= 64u, 0)) RG_WALK7_REFILL(k);
             const int l = k - k0;
-            const int skip = __builtin_amdgcn_readlane(wb.y, l);
+            const int skip = __builtin_amdgcn_readlane(wb.y, l) + k0;
             if (skip <= lo) {
                 if (next <= k) next = skip;
             } else {
@@ -1793,16 +1794,20 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             k = __ballot(next <= k + 1) ? k + 1 : skip;
         }
         // The chunk's own slots: every node a lane reaches is tested, counted
-        // and, when accepted, added.  The window check is at the end of a
-        // step, on the next slot, whose window offset l it leaves for the
-        // next step.
+        // and, when accepted, added.  Slots are counted from the window's
+        // first slot k0 here (l for the slot, nr for a lane's next, hr for
+        // the chunk end), so a step needs no offset; a refill moves all three
+        // by the shift.  The window check is at the end of a step, on the
+        // next slot.
         if (k < hi) {
             if (static_cast<unsigned>(k - k0) >= 64u) RG_WALK7_REFILL(k);
             int l = k - k0;
+            int nr = next - k0;
+            int hr = hi - k0;
             do {
                 float cx, cy, cz, crit;
                 if (LW) {
-                    const v4f r = RG_WALK7_READ(k);
+                    const v4f r = *(const lds_v4f *)(size_t)(rec0 + 16u * static_cast<unsigned>(l));
                     cx = r.x;
                     cy = r.y;
                     cz = r.z;
@@ -1816,30 +1821,16 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 const int skip = __builtin_amdgcn_readlane(wb.y, l);
                 // The test is a select; only the force and the accepting
                 // lanes' next slot are under a branch.
-                const bool act = next <= k;
+                const bool act = nr <= l;
                 const unsigned long long mact = __ballot(act);
                 const float dx = cx - px;
                 const float dy = cy - py;
                 const float dz = cz - pz;
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
                 const bool lt = crit < r2;
-                // The next slot, chosen before the force (the asm keeps the
-                // scalar select there, so its latency overlaps the force):
-                // k + 1 if an active lane opened k, else skip(k).  No other
-                // lane can need k + 1: a lane waiting at k + 1 waits at the
-                // skip of an ancestor of k whose subtree ends at k, so k is a
-                // leaf, and then skip(k) = k + 1 as well.  (s_andn2 sets SCC
-                // when its result is nonzero: no compare.)
-                int kn;
-                {
-                    unsigned long long t_;
-                    asm volatile("s_andn2_b64 %1, %2, %3\n\ts_cselect_b32 %0, %4, %5"
-                                 : "=s"(kn), "=&s"(t_)
-                                 : "s"(mact), "s"(__ballot(lt)), "s"(k + 1), "s"(skip)
-                                 : "scc");
-                }
+                const unsigned long long mlt = __ballot(lt);
                 if (act && lt) {
-                    next = skip;  // opened: stays active (next <= k + 1)
+                    nr = skip;
                     const float cw =
                         __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
                     const float inv = __builtin_amdgcn_rsqf(r2);
@@ -1849,15 +1840,31 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     az = fmaf(s, dz, az);
                     ++accepts;
                 }
-                l = kn - k0;
+                // The next slot, chosen after the force (so it can take the
+                // register of l, whose last use is there): l + 1 if an active
+                // lane opened l, else skip(l).  No other lane can need l + 1:
+                // a lane waiting at l + 1 waits at the skip of an ancestor of
+                // l whose subtree ends at l, so l is a leaf, and then
+                // skip(l) = l + 1 as well.  (s_andn2 sets SCC when its result
+                // is nonzero: no compare.)
+                int kn;
+                {
+                    unsigned long long t_;
+                    asm volatile("s_andn2_b64 %1, %2, %3\n\ts_cselect_b32 %0, %4, %5"
+                                 : "=s"(kn), "=&s"(t_)
+                                 : "s"(mact), "s"(mlt), "s"(l + 1), "s"(skip)
+                                 : "scc");
+                }
+                l = kn;
                 if (__builtin_expect(static_cast<unsigned>(l) >= 64u, 0)) {
-                    if (kn < hi) {
-                        RG_WALK7_REFILL(kn);
+                    if (l < hr) {
+                        RG_WALK7_REFILL(k0 + l);
+                        nr -= l;
+                        hr -= l;
                         l = 0;
                     }
                 }
-                k = kn;
-            } while (k < hi);
+            } while (l < hr);
         }
         s_part[c][0][lane] = ax;
         s_part[c][1][lane] = ay;
