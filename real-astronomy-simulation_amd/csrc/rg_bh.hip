@@ -1804,10 +1804,13 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             int l = k - k0;
             int nr = next - k0;
             int hr = hi - k0;
+            // LW: the record of the step's slot, read during the step before
+            v4f rn = {0.f, 0.f, 0.f, 0.f};
+            if (LW) rn = *(const lds_v4f *)(size_t)(rec0 + 16u * static_cast<unsigned>(l));
             do {
                 float cx, cy, cz, crit;
                 if (LW) {
-                    const v4f r = *(const lds_v4f *)(size_t)(rec0 + 16u * static_cast<unsigned>(l));
+                    const v4f r = rn;
                     cx = r.x;
                     cy = r.y;
                     cz = r.z;
@@ -1829,20 +1832,9 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
                 const bool lt = crit < r2;
                 const unsigned long long mlt = __ballot(lt);
-                if (act && lt) {
-                    nr = skip;
-                    const float cw =
-                        __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float s = cw * (inv * inv) * inv;
-                    ax = fmaf(s, dx, ax);
-                    ay = fmaf(s, dy, ay);
-                    az = fmaf(s, dz, az);
-                    ++accepts;
-                }
-                // The next slot, chosen after the force (so it can take the
-                // register of l, whose last use is there): l + 1 if an active
-                // lane opened l, else skip(l).  No other lane can need l + 1:
+                // The next slot, chosen before the force so that the next
+                // record's LDS read overlaps it: l + 1 if an active lane
+                // opened l, else skip(l).  No other lane can need l + 1:
                 // a lane waiting at l + 1 waits at the skip of an ancestor of
                 // l whose subtree ends at l, so l is a leaf, and then
                 // skip(l) = l + 1 as well.  (s_andn2 sets SCC when its result
@@ -1855,6 +1847,21 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                                  : "s"(mact), "s"(mlt), "s"(l + 1), "s"(skip)
                                  : "scc");
                 }
+                // the next record (a slot past the window reads slot 63,
+                // unused: the refill reads its slot 0 again)
+                if (LW)
+                    rn = *(const lds_v4f *)(size_t)(rec0 + 16u * min(static_cast<unsigned>(kn), 63u));
+                if (act && lt) {
+                    nr = skip;
+                    const float cw =
+                        __int_as_float(__builtin_amdgcn_readlane(wb.x, l));
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float s = cw * (inv * inv) * inv;
+                    ax = fmaf(s, dx, ax);
+                    ay = fmaf(s, dy, ay);
+                    az = fmaf(s, dz, az);
+                    ++accepts;
+                }
                 l = kn;
                 if (__builtin_expect(static_cast<unsigned>(l) >= 64u, 0)) {
                     if (l < hr) {
@@ -1862,6 +1869,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                         nr -= l;
                         hr -= l;
                         l = 0;
+                        if (LW) rn = *(const lds_v4f *)(size_t)rec0;
                     }
                 }
             } while (l < hr);
