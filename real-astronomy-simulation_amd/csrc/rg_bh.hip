@@ -1635,7 +1635,29 @@ struct ChunkShape {
     int run;    // XCD run length (groups)
 };
 
-template <int NW, bool LW = false>
+// The union walk's next slot: l + 1 if an active lane opened slot l, else
+// skip(l).  No other lane can need l + 1: a lane waiting at l + 1 waits at the
+// skip of an ancestor of l whose subtree ends at l, so l is a leaf, and then
+// skip(l) = l + 1 as well.  (s_andn2 sets SCC when its result is nonzero: no
+// compare.)
+__device__ __forceinline__ int walk_next_slot(unsigned long long mact, unsigned long long mlt,
+                                              int l1, int skip)
+{
+    int kn;
+    unsigned long long t_;
+    asm volatile("s_andn2_b64 %1, %2, %3\n\ts_cselect_b32 %0, %4, %5"
+                 : "=s"(kn), "=&s"(t_)
+                 : "s"(mact), "s"(mlt), "s"(l1), "s"(skip)
+                 : "scc");
+    return kn;
+}
+
+// PF (with LW): the next record's LDS read is issued before the force, so its
+// latency overlaps the force, for two more scalar instructions a step (the
+// slot clamp into the window, and a copy of the slot the force still reads).
+// That pays where the walk's waves are few per SIMD for much of the launch
+// (below ~350k particles) and costs where it is issue-bound.
+template <int NW, bool LW = false, bool PF = false>
 __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     const float4 *__restrict__ recA, const int2 *__restrict__ recB, const int *__restrict__ pre,
     const int *__restrict__ order, AccOut out, int n, float eps2,
@@ -1804,13 +1826,15 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
             int l = k - k0;
             int nr = next - k0;
             int hr = hi - k0;
-            // LW: the record of the step's slot, read during the step before
+            // LW and PF: the record of the step's slot, read during the step
+            // before
             v4f rn = {0.f, 0.f, 0.f, 0.f};
-            if (LW) rn = *(const lds_v4f *)(size_t)(rec0 + 16u * static_cast<unsigned>(l));
+            if (LW && PF) rn = *(const lds_v4f *)(size_t)(rec0 + 16u * static_cast<unsigned>(l));
             do {
                 float cx, cy, cz, crit;
                 if (LW) {
-                    const v4f r = rn;
+                    const v4f r =
+                        PF ? rn : *(const lds_v4f *)(size_t)(rec0 + 16u * static_cast<unsigned>(l));
                     cx = r.x;
                     cy = r.y;
                     cz = r.z;
@@ -1832,25 +1856,16 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                 const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
                 const bool lt = crit < r2;
                 const unsigned long long mlt = __ballot(lt);
-                // The next slot, chosen before the force so that the next
-                // record's LDS read overlaps it: l + 1 if an active lane
-                // opened l, else skip(l).  No other lane can need l + 1:
-                // a lane waiting at l + 1 waits at the skip of an ancestor of
-                // l whose subtree ends at l, so l is a leaf, and then
-                // skip(l) = l + 1 as well.  (s_andn2 sets SCC when its result
-                // is nonzero: no compare.)
-                int kn;
-                {
-                    unsigned long long t_;
-                    asm volatile("s_andn2_b64 %1, %2, %3\n\ts_cselect_b32 %0, %4, %5"
-                                 : "=s"(kn), "=&s"(t_)
-                                 : "s"(mact), "s"(mlt), "s"(l + 1), "s"(skip)
-                                 : "scc");
-                }
-                // the next record (a slot past the window reads slot 63,
-                // unused: the refill reads its slot 0 again)
-                if (LW)
+                // The next slot: with PF before the force, followed by the
+                // next record's read (a slot past the window reads slot 63,
+                // unused: the refill reads its slot 0 again); else after the
+                // force, where l has had its last use, so the new slot can
+                // take l's register.
+                int kn = 0;
+                if (LW && PF) {
+                    kn = walk_next_slot(mact, mlt, l + 1, skip);
                     rn = *(const lds_v4f *)(size_t)(rec0 + 16u * min(static_cast<unsigned>(kn), 63u));
+                }
                 if (act && lt) {
                     nr = skip;
                     const float cw =
@@ -1862,6 +1877,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                     az = fmaf(s, dz, az);
                     ++accepts;
                 }
+                if (!(LW && PF)) kn = walk_next_slot(mact, mlt, l + 1, skip);
                 l = kn;
                 if (__builtin_expect(static_cast<unsigned>(l) >= 64u, 0)) {
                     if (l < hr) {
@@ -1869,7 +1885,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
                         nr -= l;
                         hr -= l;
                         l = 0;
-                        if (LW) rn = *(const lds_v4f *)(size_t)rec0;
+                        if (LW && PF) rn = *(const lds_v4f *)(size_t)rec0;
                     }
                 }
             } while (l < hr);
@@ -2580,11 +2596,14 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
                 k_walk32_chunked2<4><<<ng, 256, smem, st>>>(h->recA, h->recB, h->pre, order, out,
                                                             n, e2, h->stats, cs);
             else if (mode == 7 && waves == 8)
-                k_walk32_chunked<8, true><<<ng, 512, smem, st>>>(h->recA, h->recB, h->pre, order,
-                                                                 out, n, e2, h->stats, cs);
+                k_walk32_chunked<8, true, true><<<ng, 512, smem, st>>>(
+                    h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
+            else if (mode == 7 && n < 350000)
+                k_walk32_chunked<4, true, true><<<ng, 256, smem, st>>>(
+                    h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
             else if (mode == 7)
-                k_walk32_chunked<4, true><<<ng, 256, smem, st>>>(h->recA, h->recB, h->pre, order,
-                                                                 out, n, e2, h->stats, cs);
+                k_walk32_chunked<4, true, false><<<ng, 256, smem, st>>>(
+                    h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
             else if (waves == 8)
                 k_walk32_chunked<8><<<ng, 512, smem, st>>>(h->recA, h->recB, h->pre, order, out,
                                                            n, e2, h->stats, cs);

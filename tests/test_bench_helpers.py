@@ -63,7 +63,7 @@ def test_walk_salu_issue_from_profile(bench):
     committed summary over the launch time, against one scalar instruction
     per cycle per CU (256 CUs at 2.4 GHz)."""
     import json
-    with open(os.path.join(ROOT, "profiles", "k_walk32_chunked4_true_diskbulge120k_sq.json")) as f:
+    with open(os.path.join(ROOT, "profiles", "k_walk32_chunked4_true_true_diskbulge120k_sq.json")) as f:
         rec = json.load(f)
     s = bench.salu_issue(rec, 0.3)
     assert s["peak"] == pytest.approx(614.4)
