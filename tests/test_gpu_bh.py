@@ -226,6 +226,37 @@ def test_fp32_walk_modes_agree(name):
         assert np.array_equal(res[6][0], res[mode][0]), mode
 
 
+@pytest.mark.parametrize("n_disk,waves", [(5000, 4), (5000, 8), (100000, 0), (100000, 8),
+                                          (333334, 0)])
+def test_chunked_walk_forms_agree(n_disk, waves):
+    """The LDS-window chunked walk comes in two forms (the next record read
+    before the force, used with 8 waves a group and below 350k particles, and
+    read at the top of the step above that) and with 4 or 8 waves a group.
+    Each is the walk of the register-window chunked walk: bitwise equal
+    accelerations, equal visit counts."""
+    from rgsim import _lib
+    from rgsim.ic import make_disk_bulge
+    s = make_disk_bulge(n_disk=n_disk, n_bulge=n_disk // 5, seed=3)
+    print(f"chunked walk forms n={s.n} waves={waves}", flush=True)
+    res = {}
+    _lib.set_param("chunk_waves", waves)
+    try:
+        for mode in (6, 7):
+            _lib.set_param("walk_mode", mode)
+            bh, acc = run_bh(s.pos, s.mass, theta=0.5, eps=0.05, precision="fp32")
+            res[mode] = (acc, bh.stats())
+            bh.close()
+    finally:
+        _lib.set_param("walk_mode", -1)
+        _lib.set_param("chunk_waves", 0)
+    (a6, s6), (a7, s7) = res[6], res[7]
+    assert s7["walk_mode"] == 7
+    want = waves if waves else (8 if s.n < 90000 else 4)
+    assert s7["walk_waves"] == want
+    assert s6["visits_total"] == s7["visits_total"] and s6["visits_max"] == s7["visits_max"]
+    assert np.array_equal(a6, a7)
+
+
 def test_bh_1m_keys_order_topology_bitexact(orc):
     """The N=1,000,000 treecode configuration (disk:bulge 5:1, seed 1):
     Morton keys, stable order and the full Karras topology equal the oracle
