@@ -50,7 +50,7 @@ WALK_RECORD_BYTES = 32          # FP32 node record fetched per visit
 # FP32 walk variant (rg_bh_stats walk_mode) -> the kernel it launches
 WALK_KERNELS = {0: "k_walk32", 1: "k_walk32_pair", 2: "k_walk32_wave", 3: "k_walk32_wave2",
                 4: "k_walk32_wave_split<2>", 5: "k_walk32_wave_split<4>",
-                6: "k_walk32_chunked<{w}>", 7: "k_walk32_chunked<{w}, true, {pf}>",
+                6: "k_walk32_chunked<{w}>", 7: "k_walk32_chunked<{w}, true, true>",
                 8: "k_walk32_chunked2<{w}>", -1: "k_walk64"}
 WALK_PARTICLE_BYTES = 16 + 4 + 24  # own leaf record + order index + FP64 acceleration
 HILO_KD_BYTES = (24 + 12 + 24 + 24) + (24 + 12 + 24)  # hi/lo kick + drift, read + written
@@ -134,10 +134,7 @@ def bench_bh(steps, warmup):
     walk_ms = tm["walk_ms"] / max(tm["count"], 1)
     walk_bytes = visits * WALK_RECORD_BYTES + n * WALK_PARTICLE_BYTES
     achieved = walk_bytes / (walk_ms * 1e-3) / 1e9
-    # walk 7 reads the next record before the force with 8 waves a group or
-    # below 350k particles
-    pf = "true" if (st_last["walk_waves"] == 8 or n < 350000) else "false"
-    kernel = WALK_KERNELS[st_last["walk_mode"]].format(w=st_last["walk_waves"], pf=pf)
+    kernel = WALK_KERNELS[st_last["walk_mode"]].format(w=st_last["walk_waves"])
     traffic = None
     fname = re.sub(r"[^A-Za-z0-9_]", "", kernel.replace(", ", "_"))
     pmc = os.path.join(ROOT, "profiles", f"{fname}_diskbulge120k_pmc.json")
@@ -647,7 +644,7 @@ def bench_bh_1m(steps=5):
            "force_err_reference": f"FP64 direct sum (the reference's arithmetic), {ref_s:.2f} s"}
     # bytes past the L2 per walk launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
     # copy-scaled, committed summary) over the measured walk time
-    pmc = os.path.join(ROOT, "profiles", "k_walk32_chunked4_true_false_diskbulge1M_pmc.json")
+    pmc = os.path.join(ROOT, "profiles", "k_walk32_chunked4_true_true_diskbulge1M_pmc.json")
     if os.path.exists(pmc) and st["walk_mode"] == 7 and st["walk_waves"] == 4:
         with open(pmc) as f:
             rec = json.load(f)

@@ -116,13 +116,15 @@ const char *rg_last_error(void);
  *                 preorder that the group's waves take in turn, 7 walk 6
  *                 with its record window in LDS (same results as 6, bitwise),
  *                 8 walk 7 with two chunks in flight per wave (same results)
- *   "chunk_dmin", "chunk_shift", "chunk_run", "chunk_waves"
+ *   "chunk_dmin", "chunk_shift", "chunk_run", "chunk_waves", "chunk_read_ahead"
  *                 shape of walks 6 and 7: leaf distance of the nearest chunk
  *                 boundary beyond each end of a group (default 256); growth
  *                 factor 2^shift of the next ones (0 = by n: 1 below 350k
  *                 particles, else 2); runs of groups per XCD (32); waves per
  *                 group, 4 or 8, or 2 for walk 8 (0 = by n: 8 below 90k
- *                 particles, and below 300k with 10-bit keys)
+ *                 particles, and below 300k with 10-bit keys); walk 7 reading
+ *                 the next record before the force: -1 (default) or 1 on,
+ *                 0 off (same results)
  * Process-wide; takes effect at the next launch. */
 int rg_set_param(const char *name, int64_t value);
 

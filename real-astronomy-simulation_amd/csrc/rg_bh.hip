@@ -58,6 +58,7 @@ int g_chunk_dmin = 256;
 int g_chunk_shift = 0;
 int g_chunk_run = 32;
 int g_chunk_waves = 0;
+int g_chunk_read_ahead = -1;  // walk 7's read-ahead form: -1 (default) or 1 on, 0 off
 
 struct alignas(16) WalkRec64 {
     double x, y, z, m;   // centre of mass, mass
@@ -1655,8 +1656,6 @@ __device__ __forceinline__ int walk_next_slot(unsigned long long mact, unsigned 
 // PF (with LW): the next record's LDS read is issued before the force, so its
 // latency overlaps the force, for two more scalar instructions a step (the
 // slot clamp into the window, and a copy of the slot the force still reads).
-// That pays where the walk's waves are few per SIMD for much of the launch
-// (below ~350k particles) and costs where it is issue-bound.
 template <int NW, bool LW = false, bool PF = false>
 __global__ __launch_bounds__(64 * NW) void k_walk32_chunked(
     const float4 *__restrict__ recA, const int2 *__restrict__ recB, const int *__restrict__ pre,
@@ -2283,6 +2282,10 @@ int rg::rg_bh_set_chunk_impl(const char *what, int64_t v)
     } else if (std::strcmp(what, "run") == 0) {
         RG_REQUIRE(v >= 1 && v <= 4096, RG_ERR_ARG, "chunk_run must be in [1, 4096]");
         g_chunk_run = static_cast<int>(v);
+    } else if (std::strcmp(what, "read_ahead") == 0) {
+        RG_REQUIRE(v >= -1 && v <= 1, RG_ERR_ARG,
+                   "chunk_read_ahead must be -1 (auto), 0 (off) or 1 (on)");
+        g_chunk_read_ahead = static_cast<int>(v);
     } else if (std::strcmp(what, "waves") == 0) {
         RG_REQUIRE(v == 0 || v == 2 || v == 4 || v == 8, RG_ERR_ARG,
                    "chunk_waves must be 0 (auto), 2 (walk 8 only), 4 or 8");
@@ -2595,15 +2598,23 @@ int bh_compute(rg_bh *h, const double *pos, const double *mass, const AccOut &ou
             else if (mode == 8)
                 k_walk32_chunked2<4><<<ng, 256, smem, st>>>(h->recA, h->recB, h->pre, order, out,
                                                             n, e2, h->stats, cs);
-            else if (mode == 7 && waves == 8)
-                k_walk32_chunked<8, true, true><<<ng, 512, smem, st>>>(
-                    h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
-            else if (mode == 7 && n < 350000)
-                k_walk32_chunked<4, true, true><<<ng, 256, smem, st>>>(
-                    h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
-            else if (mode == 7)
-                k_walk32_chunked<4, true, false><<<ng, 256, smem, st>>>(
-                    h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
+            else if (mode == 7) {
+                // the read-ahead form unless switched off (measured no slower
+                // from 120k to 10^6 particles, faster between)
+                const bool pf = g_chunk_read_ahead != 0;
+                if (waves == 8 && pf)
+                    k_walk32_chunked<8, true, true><<<ng, 512, smem, st>>>(
+                        h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
+                else if (waves == 8)
+                    k_walk32_chunked<8, true, false><<<ng, 512, smem, st>>>(
+                        h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
+                else if (pf)
+                    k_walk32_chunked<4, true, true><<<ng, 256, smem, st>>>(
+                        h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
+                else
+                    k_walk32_chunked<4, true, false><<<ng, 256, smem, st>>>(
+                        h->recA, h->recB, h->pre, order, out, n, e2, h->stats, cs);
+            }
             else if (waves == 8)
                 k_walk32_chunked<8><<<ng, 512, smem, st>>>(h->recA, h->recB, h->pre, order, out,
                                                            n, e2, h->stats, cs);

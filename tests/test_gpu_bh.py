@@ -226,20 +226,22 @@ def test_fp32_walk_modes_agree(name):
         assert np.array_equal(res[6][0], res[mode][0]), mode
 
 
-@pytest.mark.parametrize("n_disk,waves", [(5000, 4), (5000, 8), (100000, 0), (100000, 8),
-                                          (333334, 0)])
-def test_chunked_walk_forms_agree(n_disk, waves):
+@pytest.mark.parametrize("n_disk,waves,ahead", [(5000, 4, -1), (5000, 8, -1), (5000, 4, 0),
+                                                (5000, 8, 0), (100000, 0, -1), (100000, 8, -1),
+                                                (100000, 0, 0), (333334, 0, -1), (333334, 0, 0)])
+def test_chunked_walk_forms_agree(n_disk, waves, ahead):
     """The LDS-window chunked walk comes in two forms (the next record read
-    before the force, used with 8 waves a group and below 350k particles, and
-    read at the top of the step above that) and with 4 or 8 waves a group.
+    before the force, the default, or at the top of the step) and with 4 or 8
+    waves a group.
     Each is the walk of the register-window chunked walk: bitwise equal
     accelerations, equal visit counts."""
     from rgsim import _lib
     from rgsim.ic import make_disk_bulge
     s = make_disk_bulge(n_disk=n_disk, n_bulge=n_disk // 5, seed=3)
-    print(f"chunked walk forms n={s.n} waves={waves}", flush=True)
+    print(f"chunked walk forms n={s.n} waves={waves} read_ahead={ahead}", flush=True)
     res = {}
     _lib.set_param("chunk_waves", waves)
+    _lib.set_param("chunk_read_ahead", ahead)
     try:
         for mode in (6, 7):
             _lib.set_param("walk_mode", mode)
@@ -249,6 +251,7 @@ def test_chunked_walk_forms_agree(n_disk, waves):
     finally:
         _lib.set_param("walk_mode", -1)
         _lib.set_param("chunk_waves", 0)
+        _lib.set_param("chunk_read_ahead", -1)
     (a6, s6), (a7, s7) = res[6], res[7]
     assert s7["walk_mode"] == 7
     want = waves if waves else (8 if s.n < 90000 else 4)
